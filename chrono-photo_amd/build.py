@@ -1,0 +1,65 @@
+"""Builds the HIP libraries of this package in-tree for gfx950.
+
+libchrono_hip.so   -- the compositing library behind include/chrono_hip.h
+libchrono_synth.so -- seeded synthetic stacks (tests and benchmarks)
+
+Every kernel translation unit is compiled with the flags that keep the reference's f32
+semantics: no FMA contraction, correctly rounded division and square root, no denormal
+flushing, and no packed-f32 instruction selection.
+"""
+import os
+import subprocess
+import sys
+import concurrent.futures as cf
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+OUT = os.path.join(HERE, "lib")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
+    "-ffp-contract=off", "-fhip-fp32-correctly-rounded-divide-sqrt",
+    "-fno-gpu-flush-denormals-to-zero", "-fno-fast-math",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-fvisibility=hidden", "-Wall", "-Wno-unused-function",
+]
+
+LIBS = {
+    "libchrono_hip.so": ["abi.hip", "kernels.hip", "shake.hip", "batch.hip"],
+    "libchrono_synth.so": ["synth.hip"],
+}
+
+
+def _compile(src):
+    obj = os.path.join(OUT, "obj", os.path.basename(src) + ".o")
+    cmd = [HIPCC, *FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    lines = [l for l in (r.stdout + r.stderr).splitlines()
+             if "packed-fp32-ops" not in l or "error" in l]
+    if r.returncode != 0:
+        raise RuntimeError(f"compile failed: {' '.join(cmd)}\n" + "\n".join(lines))
+    return obj
+
+
+def build(verbose=False):
+    os.makedirs(os.path.join(OUT, "obj"), exist_ok=True)
+    srcs = sorted({s for v in LIBS.values() for s in v})
+    with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
+        objs = dict(zip(srcs, ex.map(_compile, srcs)))
+    for lib, parts in LIBS.items():
+        out = os.path.join(OUT, lib)
+        cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", out + ".tmp",
+               *[objs[p] for p in parts]]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}{r.stderr}")
+        os.replace(out + ".tmp", out)
+        if verbose:
+            print("built", out)
+    return OUT
+
+
+if __name__ == "__main__":
+    build(verbose=True)
+    sys.exit(0)

@@ -1,0 +1,52 @@
+// kernels.hip -- kernel instantiation and launch policy for the compositing modes.
+#include "k_outlier_generic.h"
+#include "k_simple.h"
+#include "runtime.h"
+
+namespace cp {
+
+template <int B>
+static int launch_outlier_generic_b(const cp_stack* st, const OutlierK& k, hipStream_t s) {
+    dim3 block(256), grid((k.out_w + 255) / 256, k.out_h);
+    switch (st->channels) {
+    case 1: hipLaunchKernelGGL((outlier_generic_kernel<B, 1>), grid, block, 0, s, k); break;
+    case 2: hipLaunchKernelGGL((outlier_generic_kernel<B, 2>), grid, block, 0, s, k); break;
+    case 3: hipLaunchKernelGGL((outlier_generic_kernel<B, 3>), grid, block, 0, s, k); break;
+    default: hipLaunchKernelGGL((outlier_generic_kernel<B, 4>), grid, block, 0, s, k); break;
+    }
+    return CP_OK;
+}
+
+int launch_outlier(cp_ctx* ctx, const cp_stack* st, const OutlierK& k, uint32_t flags,
+                   hipStream_t s) {
+    (void)ctx;
+    (void)flags;
+    if (st->bps == 1) launch_outlier_generic_b<1>(st, k, s);
+    else launch_outlier_generic_b<2>(st, k, s);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(CP_ERR_HIP, "outlier kernel launch: %s", hipGetErrorString(e));
+    return CP_OK;
+}
+
+template <int B, int PX>
+static void launch_simple_b(const cp_stack* st, const SimpleK& k, hipStream_t s) {
+    const uint32_t lanes = (k.out_w + PX - 1) / PX;
+    dim3 block(256), grid((lanes + 255) / 256, k.out_h);
+    switch (st->channels) {
+    case 1: hipLaunchKernelGGL((simple_kernel<B, 1, PX>), grid, block, 0, s, k); break;
+    case 2: hipLaunchKernelGGL((simple_kernel<B, 2, PX>), grid, block, 0, s, k); break;
+    case 3: hipLaunchKernelGGL((simple_kernel<B, 3, PX>), grid, block, 0, s, k); break;
+    default: hipLaunchKernelGGL((simple_kernel<B, 4, PX>), grid, block, 0, s, k); break;
+    }
+}
+
+int launch_simple(cp_ctx* ctx, const cp_stack* st, const SimpleK& k, hipStream_t s) {
+    (void)ctx;
+    if (st->bps == 1) launch_simple_b<1, 4>(st, k, s);
+    else launch_simple_b<2, 2>(st, k, s);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(CP_ERR_HIP, "simple kernel launch: %s", hipGetErrorString(e));
+    return CP_OK;
+}
+
+}  // namespace cp

@@ -1,0 +1,87 @@
+// runtime.h -- host-side context, stack and parameter staging shared by the ABI sources.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/chrono_hip.h"
+#include "kparams.h"
+
+namespace cp {
+
+void set_error(const char* fmt, ...);
+int fail(int code, const char* fmt, ...);
+
+#define CP_HIP_TRY(expr)                                                                  \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            return ::cp::fail(CP_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+// Ring of small per-call parameter blocks: pinned host staging + device mirror, copied
+// asynchronously on the owning stream.  When the ring wraps the stream is drained first,
+// so a block is never rewritten while a queued copy or kernel may still read it.
+struct ParamArena {
+    uint8_t* host = nullptr;
+    uint8_t* dev = nullptr;
+    size_t cap = 0, head = 0;
+    int init(size_t bytes);
+    void release();
+    // returns device pointer of a copy of `data`, or nullptr on failure
+    void* put(const void* data, size_t bytes, hipStream_t s);
+};
+
+struct DeviceBuffer {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+
+}  // namespace cp
+
+struct cp_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint64_t budget = 0;
+    uint64_t used = 0;
+    int cu_count = 0;
+    cp::ParamArena arena;
+    cp::DeviceBuffer out_px, out_mask, out_stats, warn;  // host-output staging
+    std::mutex mu;
+    int ensure(cp::DeviceBuffer& b, size_t bytes);
+    int charge(uint64_t bytes);
+    void uncharge(uint64_t bytes) { used = used >= bytes ? used - bytes : 0; }
+};
+
+struct cp_stack {
+    cp_ctx* ctx = nullptr;
+    uint32_t width = 0, height = 0, channels = 0, bps = 0;
+    uint32_t capacity = 0, count = 0;
+    uint8_t* dev = nullptr;
+    bool owned = false;
+    uint64_t frame_bytes() const { return (uint64_t)width * height * channels * bps; }
+    uint64_t frame_elems() const { return (uint64_t)width * height * channels; }
+};
+
+namespace cp {
+
+// Resolved geometry of one composite (shared by every mode).
+struct Geometry {
+    uint32_t n = 0;
+    uint32_t out_w = 0, out_h = 0;
+    std::vector<uint64_t> pos_off;  // element offset per frame position
+};
+
+int resolve_frames(const cp_stack* st, const cp_frames* fr, Geometry* g);
+FadeK fade_to_device(cp_ctx* ctx, const cp_fade* f, int* rc);
+
+// Kernel launchers (defined in the kernel translation units).
+int launch_outlier(cp_ctx* ctx, const cp_stack* st, const OutlierK& k, uint32_t flags,
+                   hipStream_t s);
+int launch_simple(cp_ctx* ctx, const cp_stack* st, const SimpleK& k, hipStream_t s);
+
+}  // namespace cp

@@ -1,0 +1,318 @@
+"""Pins of the CPU oracle (oracle/chrono_oracle.c) against the reference's own known
+answers, closed forms, invariants and brute force.  CPU only."""
+import numpy as np
+import pytest
+
+import oracle_py as O
+
+F32 = np.float32
+
+
+def rround(x):
+    """Rust f32::round (half away from zero), elementwise on float32."""
+    x = np.asarray(x, np.float32)
+    return np.where(x >= 0, np.floor(x + F32(0.5)), np.ceil(x - F32(0.5))).astype(np.float32)
+
+
+# --------------------------------------------------------------- order statistics
+
+def test_quartiles_reference_kat():
+    # src/chrono.rs:594-605 quartiles_test: [0..6] -> (1.0, 3.0, 5.0)
+    d = np.arange(7, dtype=np.uint32)
+    assert (O.quantile(d, 0.25), O.median(d), O.quantile(d, 0.75)) == (1.0, 3.0, 5.0)
+
+
+@pytest.mark.parametrize("bits", [8, 16])
+def test_quantiles_match_hyndman_fan_type6(bits):
+    rng = np.random.default_rng(7 + bits)
+    for n in list(range(3, 80)) + [127, 128, 255, 256, 399]:
+        x = np.sort(rng.integers(0, 1 << bits, n)).astype(np.uint32)
+        q = np.quantile(x.astype(np.float64), [0.25, 0.5, 0.75], method="weibull")
+        got = (O.quantile(x, 0.25), O.median(x), O.quantile(x, 0.75))
+        assert got == tuple(float(v) for v in q), (n, got, q)
+
+
+def test_median_small_and_even():
+    assert O.median(np.array([5], np.uint32)) == 5.0
+    assert O.median(np.array([1, 4], np.uint32)) == 2.5
+    assert O.median(np.array([0, 255, 255], np.uint32)) == 255.0
+
+
+# ---------------------------------------------------------------- option values
+
+def test_threshold_new_closed_form():
+    t = O.threshold(True, 0.05, 0.2)
+    assert t.min == F32(F32(0.05) * F32(255.0))
+    assert t.max == F32(F32(0.2) * F32(255.0))
+    assert t.scale == F32(F32(1.0) / F32((F32(0.2) - F32(0.05)) * F32(255.0)))
+    r = O.threshold(False, 2.0, 4.0)
+    assert (r.min, r.max, r.scale) == (2.0, 4.0, 0.5)
+    w = O.threshold(True, 0.05, 0.2, bps=2)
+    assert w.min == F32(F32(0.05) * F32(65535.0))
+    lib = O.lib()
+    assert lib.or_blend_value(t, 12.0) == 0.0
+    assert lib.or_blend_value(t, 60.0) == 1.0
+    mid = lib.or_blend_value(t, 30.0)
+    assert mid == F32((F32(30.0) - t.min) * t.scale)
+    single = O.threshold(True, 0.1, 0.1)  # max omitted: max == min, scale = inf, never used
+    assert lib.or_blend_value(single, single.min) == 0.0
+    assert lib.or_blend_value(single, np.nextafter(single.min, np.float32(1e9))) == 1.0
+
+
+def test_fade_table_linear_and_modes():
+    tab = O.fade_table([0, 10], [0.0, 1.0])
+    assert len(tab) == 11
+    for i in range(11):
+        assert tab[i] == F32(F32(0.0) + F32(F32(1.0) - F32(0.0)) * F32(i) / F32(10))
+    tab = O.fade_table([2, 4, 8], [1.0, 0.0, 0.5])
+    assert list(tab[:3]) == [1.0, 0.5, 0.0] and tab[-1] == 0.5 and len(tab) == 7
+    clamp = O.Fade(False, True, [2, 4], [1.0, 0.0])
+    assert clamp.get(-5) == 1.0 and clamp.get(100) == 0.0 and clamp.get(3) == 0.5
+    rep = O.Fade(True, True, [0, 3], [0.0, 0.75])  # table [0, .25, .5, .75]
+    assert [rep.get(f) for f in (-1, 4, 5, 8)] == [0.75, 0.0, 0.25, 0.0]
+    assert O.Fade.none().get(12345) == 1.0
+
+
+def test_blend_into_u8_reference_kat():
+    # commented reference test src/color.rs:51-64: blend 0 -> 255 by 0.5 gives 128
+    assert list(O.blend_into_u8([0] * 4, [255] * 4, 0.5)) == [128] * 4
+    assert list(O.blend_into_u8([0] * 4, [255] * 4, 0.0)) == [0] * 4
+    assert list(O.blend_into_u8([0] * 4, [255] * 4, 1.0)) == [255] * 4
+    assert list(O.blend_into_u8([10], [20], 0.25)) == [13]  # 12.5 rounds away from zero
+
+
+# ------------------------------------------------------------------ outlier core
+
+def _stack(rng, n, h, w, c, bits=8, outlier_frac=0.15):
+    hi = (1 << bits) - 1
+    base = rng.integers(hi // 4, 3 * hi // 4, (1, h, w, c))
+    noise = rng.integers(-hi // 40 - 1, hi // 40 + 2, (n, h, w, c))
+    x = np.clip(base + noise, 0, hi)
+    hit = rng.random((n, h, w)) < outlier_frac
+    x = np.where(hit[..., None], rng.integers(0, hi + 1, (n, h, w, c)), x)
+    return x.astype(np.uint8 if bits == 8 else np.uint16)
+
+
+def _numpy_distances(x, med, iqr_inv, w, absolute):
+    """Vectorised float32 re-evaluation of src/chrono.rs:257-278 in channel order."""
+    n, h, wd, c = x.shape
+    d = np.zeros((n, h, wd), np.float32)
+    for i in range(c):
+        wi = F32(w[i])
+        if wi == 0:
+            continue
+        diff = (med[None, :, :, i] - x[..., i].astype(np.float32)).astype(np.float32)
+        if absolute:
+            a = (wi * diff).astype(np.float32)
+        else:
+            a = ((wi * iqr_inv[None, :, :, i]).astype(np.float32) * diff).astype(np.float32)
+        term = (F32(np.copysign(1.0, wi)) * (a * a).astype(np.float32)).astype(np.float32)
+        term = np.where(diff == 0, F32(0), term)
+        d = (d + term).astype(np.float32)
+    return d
+
+
+@pytest.mark.parametrize("absolute,bits", [(True, 8), (False, 8), (True, 16), (False, 16)])
+def test_outlier_statistics_and_mask_invariants(absolute, bits):
+    rng = np.random.default_rng(11 + bits + absolute)
+    x = _stack(rng, 23, 6, 7, 3, bits)
+    thr = O.threshold(absolute, 0.05, 0.2, bps=bits // 8) if absolute else O.threshold(False, 2.0, 4.0)
+    w = (1.0, 0.5, 2.0, 1.0)
+    img, mask, warn, st = O.outlier(x, thr, 0, 2, weights=w, stats=True)
+    xs = np.sort(x.astype(np.float64), axis=0)
+    q1, med, q3 = np.quantile(xs, [0.25, 0.5, 0.75], axis=0, method="weibull")
+    assert np.array_equal(st["median"][..., :3], med.astype(np.float32))
+    if absolute:
+        assert np.all(st["iqr_inv"] == 0)
+        iqr_inv = np.zeros_like(med, np.float32)
+    else:
+        iqr = (q3 - q1).astype(np.float32)
+        iqr = np.where(iqr == 0, F32(1), iqr)
+        iqr_inv = (F32(1) / iqr).astype(np.float32)
+        assert np.array_equal(st["iqr_inv"][..., :3], iqr_inv)
+    d = _numpy_distances(x, med.astype(np.float32), iqr_inv, w, absolute)
+    out = d >= F32(thr.min) * F32(thr.min)
+    assert np.array_equal(st["num_outliers"], out.sum(axis=0))
+    # extreme = first maximum among outliers (strict > keeps the earliest frame)
+    dm = np.where(out, d, F32(0))
+    has = out.any(axis=0)
+    first_max = np.argmax(dm, axis=0)
+    assert np.array_equal(np.where(has, st["max_index"], 0), np.where(has, first_max, 0))
+    # no outliers: background first frame, blend mask 0
+    none = ~has
+    assert np.all(mask[none] == 0)
+    assert np.array_equal(img[none], x[0][none])
+
+
+def test_outlier_first_last_extreme_select_the_right_frame():
+    rng = np.random.default_rng(3)
+    x = _stack(rng, 30, 5, 6, 3, 8, outlier_frac=0.3)
+    thr = O.threshold(True, 0.05, 0.05)  # max == min: any outlier blends fully
+    for mode in (0, 1, 2):
+        img, mask, _, st = O.outlier(x, thr, 3, mode, stats=True)
+        med = np.median(x.astype(np.float64), axis=0).astype(np.float32)
+        d = _numpy_distances(x, med, np.zeros_like(med), (1, 1, 1, 1), True)
+        out = d >= F32(thr.min) * F32(thr.min)
+        for yy in range(5):
+            for xx in range(6):
+                ts = np.nonzero(out[:, yy, xx])[0]
+                if len(ts) == 0:
+                    assert np.array_equal(img[yy, xx], rround(med[yy, xx]).astype(np.uint8))
+                    continue
+                t = {0: ts[0], 1: ts[-1], 2: ts[np.argmax(d[ts, yy, xx])]}[mode]
+                assert np.array_equal(img[yy, xx], x[t, yy, xx]), (mode, yy, xx)
+                assert mask[yy, xx, 0] == 255
+
+
+def test_forward_backward_with_full_blend_end_on_last_and_first_outlier():
+    rng = np.random.default_rng(5)
+    x = _stack(rng, 25, 4, 5, 3, 8, outlier_frac=0.4)
+    thr = O.threshold(True, 0.05, 0.05)
+    f, _, _, st = O.outlier(x, thr, 0, 4, stats=True)
+    b, _, _, _ = O.outlier(x, thr, 0, 5, stats=True)
+    l, _, _ = O.outlier(x, thr, 0, 1)
+    fi, _, _ = O.outlier(x, thr, 0, 0)
+    many = st["num_outliers"] >= 1
+    assert np.array_equal(f[many], l[many])
+    assert np.array_equal(b[many], fi[many])
+
+
+def test_background_average_without_outliers_is_rounded_mean():
+    rng = np.random.default_rng(9)
+    x = rng.integers(100, 110, (9, 4, 4, 3)).astype(np.uint8)  # |diff| < 12.75: no outliers
+    img, mask, _ = O.outlier(x, O.threshold(True, 0.05, 0.2), 2, 2)
+    mean = (x.astype(np.float32).sum(axis=0) / F32(9)).astype(np.float32)
+    assert np.array_equal(img, rround(mean).astype(np.uint8))
+    assert np.all(mask[..., :3] == 0)
+
+
+def test_background_first_and_warning_for_all_outlier_pixel():
+    x = np.zeros((4, 1, 1, 3), np.uint8)
+    x[:, 0, 0, 0] = [0, 0, 255, 255]  # median 127.5: every frame is 127.5 away
+    img, mask, warn, st = O.outlier(x, O.threshold(True, 0.05, 0.2), 0, 2, stats=True)
+    assert st["num_outliers"][0, 0] == 4 and warn == 1 and st["warning"][0, 0] == 1
+
+
+def test_rel_threshold_needs_three_samples():
+    x = np.zeros((2, 1, 1, 3), np.uint8)
+    with pytest.raises(ValueError):
+        O.outlier(x, O.threshold(False, 2.0, 4.0), 0, 2)
+
+
+def test_random_background_swap_candidates_can_hold_outliers():
+    # src/chrono.rs:544-554: N=3, outliers {0, 2}: swaps (0,2) then (2,1) leave [2, 0, 1];
+    # the single candidate is index 2, an outlier (reference quirk).
+    x = np.zeros((3, 1, 1, 1), np.uint8)
+    x[:, 0, 0, 0] = [0, 100, 200]  # median 100: frames 0 and 2 are outliers
+    img, mask, warn, st = O.outlier(x, O.threshold(True, 0.05, 0.2), 1, 2, stats=True, seed=123)
+    assert st["num_outliers"][0, 0] == 2
+    assert st["bg_index"][0, 0] == 2
+
+
+def test_sample_subset_median():
+    rng = np.random.default_rng(1)
+    x = _stack(rng, 20, 3, 3, 3, 8)
+    pos = np.array([1, 4, 5, 9, 12, 17, 19], np.uint32)
+    img, mask, warn, st = O.outlier(x, O.threshold(True, 0.05, 0.2), 0, 2, sample_pos=pos, stats=True)
+    med = np.median(x[pos].astype(np.float64), axis=0).astype(np.float32)
+    assert np.array_equal(st["median"][..., :3], med)
+
+
+# --------------------------------------------------------------- darker/lighter
+
+@pytest.mark.parametrize("darker", [True, False])
+def test_simple_without_fade_is_first_argextreme(darker):
+    rng = np.random.default_rng(2 + darker)
+    x = rng.integers(0, 4, (17, 5, 6, 3)).astype(np.uint8)  # many ties
+    w = (1.0, 1.0, 1.0, 1.0)
+    img = O.simple(x, w, darker=darker)
+    v = x.astype(np.float32).sum(axis=-1)
+    idx = np.argmin(v, axis=0) if darker else np.argmax(v, axis=0)
+    exp = np.take_along_axis(x, idx[None, :, :, None], axis=0)[0]
+    assert np.array_equal(img, exp)
+
+
+def test_simple_with_fade_blends_on_each_new_extreme():
+    x = np.zeros((3, 1, 1, 1), np.uint8)
+    x[:, 0, 0, 0] = [200, 100, 50]  # each frame is a new darker extreme
+    fade = O.Fade(False, True, [0, 2], [0.5, 0.5])
+    img = O.simple(x, (1, 1, 1, 1), fade=fade, darker=True)
+    o = 0
+    for v in (200, 100, 50):
+        o = int(rround(F32(o) + (F32(v) - F32(o)) * F32(0.5)))
+    assert img[0, 0, 0] == o
+
+
+# ------------------------------------------------------------------- shake + crop
+
+def _textured_frames(n, h, w, shifts, seed=0):
+    rng = np.random.default_rng(seed)
+    world = rng.integers(0, 256, (h + 40, w + 40, 3)).astype(np.uint8)
+    out = np.zeros((n, h, w, 3), np.uint8)
+    for t, (dx, dy) in enumerate(shifts):
+        out[t] = world[20 - dy:20 - dy + h, 20 - dx:20 - dx + w]
+    return out
+
+
+def test_shake_recovers_translations_and_matches_brute_force():
+    shifts = [(0, 0), (2, -1), (-3, 3), (1, 0), (0, -2)]
+    x = _textured_frames(5, 40, 48, shifts)
+    anchors = [(16, 14), (30, 24)]
+    off = O.shake(x, anchors, 4, 4)
+    assert [tuple(o) for o in off] == shifts
+    # brute force int64 SSD, first minimum in row-major order
+    r, s = 4, 4
+    for t in range(1, 5):
+        best, bi = None, None
+        for oy in range(2 * s + 1):
+            for ox in range(2 * s + 1):
+                tot = 0
+                for cx, cy in anchors:
+                    win = x[0, cy - r:cy + r + 1, cx - r:cx + r + 1].astype(np.int64)
+                    yy, xx = cy + oy - s, cx + ox - s
+                    img = x[t, yy - r:yy + r + 1, xx - r:xx + r + 1].astype(np.int64)
+                    tot += int(((win - img) ** 2).sum())
+                if best is None or tot < best:
+                    best, bi = tot, (ox - s, oy - s)
+        assert tuple(off[t]) == bi
+
+
+def test_shake_anchor_out_of_range():
+    x = np.zeros((2, 20, 20, 3), np.uint8)
+    with pytest.raises(ValueError):
+        O.shake(x, [(5, 5)], 3, 3)
+
+
+def test_crop_create_closed_form():
+    off = [(0, 0), (2, -1), (-3, 3)]
+    crop, w, h = O.crop_create(off, 100, 80)
+    assert (w, h) == (100 - 3 - 2, 80 - 1 - 3)
+    assert crop.tolist() == [[3, 1], [5, 0], [0, 4]]
+    assert O.crop_create([(0, 0), (0, 0)], 10, 10) is None
+
+
+# -------------------------------------------------------------------- video windows
+
+def test_video_windows_full_windows():
+    # --video-in 0/60/. --video-out 0/540/. over 600 frames: 540 windows of 60 frames
+    wins = O.video_windows((0, 60, 1), (0, 540, 1), 600)
+    assert len(wins) == 540
+    assert all(len(ix) == 60 for _, ix in wins)
+    assert wins[0] == (0, list(range(0, 60)))
+    assert wins[539] == (539, list(range(539, 599)))
+
+
+def test_video_windows_default_out_ramps_and_never_uses_last_frame():
+    # --video-in 0/60/. alone: v_lower = -59, 659 windows ramping 1..60..1 frames;
+    # the end clamp min(N + (f+e)%1 - 1, f+e) keeps frame 599 out of every window
+    wins = O.video_windows((0, 60, 1), (None, None, 1), 600)
+    assert len(wins) == 659
+    sizes = [len(ix) for _, ix in wins]
+    assert sizes[0] == 1 and max(sizes) == 60 and sizes[-1] == 0
+    assert all(599 not in ix for _, ix in wins)
+
+
+def test_video_windows_step():
+    wins = O.video_windows((0, 10, 3), (0, 4, 2), 20)
+    # frames 0 and 2: start = max(st % 3, f), end = min(20 + (f+10)%3 - 3, f + 10)
+    assert wins == [(0, [0, 3, 6, 9]), (2, [2, 5, 8, 11])]

@@ -1,0 +1,220 @@
+"""GPU parity: the HIP kernels behind the C ABI against the CPU oracle, bit-exact.
+
+Integer outputs (pixels, blend masks, outlier counts, selected frames) and the f32 order
+statistics (medians, inverse IQR) must match exactly.  Sizes are small so the oracle
+finishes in well under a second per case."""
+import itertools
+
+import numpy as np
+import pytest
+
+import chrono_photo_amd as cpa
+import oracle_py as O
+
+pytestmark = pytest.mark.gpu
+
+
+def make_stack(seed, n, h, w, c, bits=8, outlier_frac=0.15):
+    rng = np.random.default_rng(seed)
+    hi = (1 << bits) - 1
+    base = rng.integers(hi // 4, 3 * hi // 4, (1, h, w, c))
+    noise = rng.integers(-(hi // 40) - 1, hi // 40 + 2, (n, h, w, c))
+    x = np.clip(base + noise, 0, hi)
+    hit = rng.random((n, h, w)) < outlier_frac
+    x = np.where(hit[..., None], rng.integers(0, hi + 1, (n, h, w, c)), x)
+    # a few all-outlier and tied pixels
+    x[:, 0, 0, :] = np.where(np.arange(n)[:, None] % 2 == 0, 0, hi)
+    x[:, 0, 1, :] = 7
+    return x.astype(np.uint8 if bits == 8 else np.uint16)
+
+
+def run_both(ctx, x, thr_abs, lo, hi, bg, sel, weights=(1, 1, 1, 1), fade=None, slots=None,
+             crop=None, sample=None, seed=42, force_generic=False):
+    bps = x.dtype.itemsize
+    st = cpa.Stack.from_array(ctx, x)
+    thr = cpa.Threshold(thr_abs, lo, hi)
+    cfade = cpa.Fade.none() if fade is None else cpa.Fade(fade[0], fade[1], fade[2])
+    proc = cpa.OutlierProcessor(thr, bg, sel, weights, cfade, sample_indices=sample, seed=seed)
+    cr = None
+    if crop is not None:
+        cr = crop
+    img, mask, stats = proc.process(st, image_indices=slots, crop=cr, with_stats=True,
+                                    force_generic=force_generic)
+    ofade = O.Fade.none() if fade is None else O.Fade(fade[0], fade[1], [f for f, _ in fade[2]],
+                                                        [v for _, v in fade[2]])
+    oimg, omask, owarn, ost = O.outlier(
+        x, O.threshold(thr_abs, lo, hi, bps), bg, sel, weights=weights, fade=ofade, seed=seed,
+        slots=slots, crop_xy=None if crop is None else crop[0],
+        out_w=None if crop is None else crop[1], out_h=None if crop is None else crop[2],
+        sample_pos=sample, stats=True)
+    return (img, mask, stats, proc.warnings), (oimg, omask, ost, owarn)
+
+
+def assert_same(gpu, ref, ctx_msg=""):
+    img, mask, st, warn = gpu
+    oimg, omask, ost, owarn = ref
+    assert np.array_equal(st["median"], ost["median"]), ctx_msg + " median"
+    assert np.array_equal(st["iqr_inv"], ost["iqr_inv"]), ctx_msg + " iqr_inv"
+    assert np.array_equal(st["num_outliers"], ost["num_outliers"]), ctx_msg + " num_outliers"
+    assert np.array_equal(st["max_index"], ost["max_index"]), ctx_msg + " max_index"
+    assert np.array_equal(st["max_dist_sq"], ost["max_dist_sq"]), ctx_msg + " max_dist_sq"
+    assert np.array_equal(st["bg_index"], ost["bg_index"]), ctx_msg + " bg_index"
+    assert np.array_equal(st["warning"], ost["warning"]), ctx_msg + " warning"
+    bad = np.argwhere(img != oimg)
+    assert bad.size == 0, f"{ctx_msg} pixels differ at {bad[:5].tolist()} gpu={img[tuple(bad[0])]} ref={oimg[tuple(bad[0])]}"
+    assert np.array_equal(mask, omask), ctx_msg + " mask"
+    assert warn == owarn, ctx_msg + " warnings"
+
+
+MODES = list(itertools.product([0, 1, 2, 3], [0, 1, 2, 3, 4, 5]))
+
+
+@pytest.mark.parametrize("bg,sel", MODES)
+@pytest.mark.parametrize("absolute", [True, False])
+def test_outlier_all_modes_u8_rgb(ctx, bg, sel, absolute):
+    x = make_stack(100 + bg * 7 + sel, 17, 13, 19, 3)
+    lo, hi = (0.05, 0.2) if absolute else (2.0, 4.0)
+    for generic in (False, True):
+        gpu, ref = run_both(ctx, x, absolute, lo, hi, bg, sel, force_generic=generic)
+        assert_same(gpu, ref, f"bg={bg} sel={sel} abs={absolute} generic={generic}")
+
+
+@pytest.mark.parametrize("sel", [0, 2, 3, 4, 5])
+@pytest.mark.parametrize("absolute", [True, False])
+def test_outlier_u16_rgb(ctx, sel, absolute):
+    x = make_stack(300 + sel, 21, 11, 14, 3, bits=16)
+    lo, hi = (0.05, 0.2) if absolute else (2.0, 4.0)
+    gpu, ref = run_both(ctx, x, absolute, lo, hi, 0, sel)
+    assert_same(gpu, ref, f"u16 sel={sel}")
+
+
+@pytest.mark.parametrize("c", [1, 2, 4])
+def test_outlier_channel_counts_and_weights(ctx, c):
+    x = make_stack(500 + c, 15, 9, 10, c)
+    for w in [(1, 1, 1, 1), (0.3, 0, 2.0, 1.0), (-1.0, 0.5, 1.0, 0.0)]:
+        gpu, ref = run_both(ctx, x, True, 0.05, 0.2, 2, 3, weights=w)
+        assert_same(gpu, ref, f"c={c} w={w}")
+        gpu, ref = run_both(ctx, x, False, 1.5, 3.0, 3, 4, weights=w)
+        assert_same(gpu, ref, f"c={c} w={w} rel")
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 8, 31, 32, 33, 64, 65, 100])
+def test_outlier_frame_counts(ctx, n):
+    x = make_stack(700 + n, n, 7, 9, 3)
+    gpu, ref = run_both(ctx, x, True, 0.05, 0.2, 0, 4)
+    assert_same(gpu, ref, f"n={n}")
+    if n >= 3:
+        gpu, ref = run_both(ctx, x, False, 2.0, 4.0, 2, 3)
+        assert_same(gpu, ref, f"n={n} rel")
+
+
+def test_outlier_fade_windows_crop_sample(ctx):
+    x = make_stack(900, 30, 12, 16, 3)
+    fade = (False, True, [(2, 0.2), (10, 1.0), (20, 0.4)])
+    slots = list(range(5, 25))
+    gpu, ref = run_both(ctx, x, True, 0.05, 0.2, 0, 4, fade=fade, slots=slots)
+    assert_same(gpu, ref, "fade abs window")
+    fade_rel = (True, False, [(0, 0.0), (7, 1.0)])
+    gpu, ref = run_both(ctx, x, True, 0.05, 0.3, 2, 5, fade=fade_rel, slots=slots)
+    assert_same(gpu, ref, "fade rel repeat")
+    rng = np.random.default_rng(1)
+    crop_xy = rng.integers(0, 4, (30, 2)).astype(np.int32)
+    gpu, ref = run_both(ctx, x, False, 2.0, 4.0, 1, 2, crop=(crop_xy, 12, 8))
+    assert_same(gpu, ref, "crop")
+    sample = [0, 3, 4, 9, 11, 12, 20, 28, 29]
+    gpu, ref = run_both(ctx, x, True, 0.05, 0.2, 3, 3, sample=sample)
+    assert_same(gpu, ref, "sample")
+    gpu, ref = run_both(ctx, x, False, 2.0, 4.0, 1, 4, sample=sample)
+    assert_same(gpu, ref, "sample rel")
+
+
+def test_outlier_rel_too_few_frames(ctx):
+    x = make_stack(1, 2, 4, 4, 3)
+    st = cpa.Stack.from_array(ctx, x)
+    proc = cpa.OutlierProcessor(cpa.Threshold.rel(2.0, 4.0), 0, 2)
+    with pytest.raises(cpa.ChronoError) as e:
+        proc.process(st)
+    assert e.value.code == cpa.CP_ERR_TOO_FEW_FRAMES_REL
+
+
+# ------------------------------------------------------------------ darker / lighter
+
+@pytest.mark.parametrize("darker", [True, False])
+@pytest.mark.parametrize("bits,c", [(8, 3), (8, 4), (16, 3), (8, 1)])
+def test_simple_parity(ctx, darker, bits, c):
+    x = make_stack(40 + bits + c, 19, 10, 23, c, bits=bits, outlier_frac=0.3)
+    st = cpa.Stack.from_array(ctx, x)
+    for w, fade in [((1, 1, 1, 1), None), ((0.2, 0.7, 0.1, 1.0), (False, True, [(0, 0.3), (18, 1.0)])),
+                    ((1, -1, 0.5, 0), (True, False, [(0, 1.0), (4, 0.0)]))]:
+        cfade = cpa.Fade.none() if fade is None else cpa.Fade(*fade)
+        ofade = O.Fade.none() if fade is None else O.Fade(fade[0], fade[1], [f for f, _ in fade[2]],
+                                                            [v for _, v in fade[2]])
+        img = cpa.SimpleProcessor(w, cfade, darker).process(st)
+        ref = O.simple(x, w, ofade, darker)
+        assert np.array_equal(img, ref), (darker, bits, c, w)
+        slots = list(range(3, 15))
+        img = cpa.SimpleProcessor(w, cfade, darker).process(st, image_indices=slots)
+        ref = O.simple(x, w, ofade, darker, slots=slots)
+        assert np.array_equal(img, ref)
+
+
+# --------------------------------------------------------------------------- shake
+
+def textured(n, h, w, shifts, c=3, bits=8, seed=0):
+    rng = np.random.default_rng(seed)
+    hi = (1 << bits) - 1
+    world = rng.integers(0, hi + 1, (h + 40, w + 40, c))
+    out = np.zeros((n, h, w, c), np.int64)
+    for t, (dx, dy) in enumerate(shifts):
+        out[t] = world[20 - dy:20 - dy + h, 20 - dx:20 - dx + w]
+    return out.astype(np.uint8 if bits == 8 else np.uint16)
+
+
+@pytest.mark.parametrize("bits", [8, 16])
+def test_shake_parity(ctx, bits):
+    shifts = [(0, 0), (2, -1), (-3, 3), (1, 0), (0, -2), (4, 4), (-4, -1)]
+    x = textured(len(shifts), 48, 64, shifts, bits=bits)
+    st = cpa.Stack.from_array(ctx, x)
+    anchors = [(20, 16), (40, 30)]
+    off = cpa.ShakeAnalyzer().analyze(st, anchors, 5, 4)
+    ref = O.shake(x, anchors, 5, 4)
+    assert np.array_equal(off, ref)
+    assert [tuple(o) for o in off] == shifts
+
+
+def test_shake_u8_wrapping_sum_matches(ctx):
+    # bright/dark contrast with big windows drives the i32 SSD past 2^31 (wraps like the
+    # reference's release build)
+    shifts = [(0, 0), (1, 1), (-2, 0)]
+    x = textured(3, 120, 120, shifts, bits=8, seed=3)
+    x[1:] = 255 - x[1:]
+    st = cpa.Stack.from_array(ctx, x)
+    anchors = [(60, 60), (50, 55), (70, 62)]
+    off = cpa.ShakeAnalyzer().analyze(st, anchors, 30, 6)
+    assert np.array_equal(off, O.shake(x, anchors, 30, 6))
+
+
+def test_shake_anchor_errors(ctx):
+    x = textured(2, 30, 30, [(0, 0), (1, 1)])
+    st = cpa.Stack.from_array(ctx, x)
+    with pytest.raises(cpa.ChronoError) as e:
+        cpa.ShakeAnalyzer().analyze(st, [(4, 4)], 3, 3)
+    assert e.value.code == cpa.CP_ERR_ANCHOR_OUT_OF_RANGE
+
+
+def test_shake_crop_outlier_pipeline(ctx):
+    """Shake offsets -> Crop::create -> outlier composite on the cropped stack."""
+    scene = cpa.Scene(seed=5, width=96, height=72, bytes_per_sample=2, n_frames=12, base_lo=8000,
+                      base_hi=50000, sigma=200, n_disks=2, disk_r_min=6, disk_r_max=10,
+                      disk_val_lo=2000, disk_val_hi=3000, shake_max=3, texture_amp=3000)
+    x = scene.generate()
+    st = cpa.Stack.from_array(ctx, x)
+    anchors = [(24, 20), (70, 50)]
+    off = cpa.ShakeAnalyzer().analyze(st, anchors, 6, 4)
+    assert np.array_equal(off, scene.shake_truth())
+    assert np.array_equal(off, O.shake(x, anchors, 6, 4))
+    crop = cpa.Crop.create(off, 96, 72)
+    ocrop = O.crop_create(off, 96, 72)
+    assert crop is not None and np.array_equal(crop[0], ocrop[0]) and crop[1:] == ocrop[1:]
+    gpu, ref = run_both(ctx, x, True, 0.05, 0.2, 0, 4, crop=crop)
+    assert_same(gpu, ref, "shake crop")
