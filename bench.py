@@ -1,0 +1,225 @@
+"""Throughput of the per-pixel stack-compositing hot path on one MI355X.
+
+Default workload (outlier mode on a 256-frame 4K RGB16 stack, shake-corrected):
+  * 256 synthetic RGB16 frames 3840x2160 (gradient 8000->50000, noise sigma 200, three
+    moving dark disks, a static world-space block texture, a seeded integer translation of
+    up to +-8 px per frame), generated directly in HBM;
+  * shake analysis with 5 anchors, anchor radius 32, search radius 12, then Crop::create;
+  * the timed step: one outlier composite (threshold abs/0.05/0.2, background first,
+    --outlier forward) over the cropped stack, image + blend mask written to HBM.
+
+One step = one pass of the hot path over the resident stack.  pixel-samples/s =
+frames * out_h * out_w / step time.  The JSON line carries the roofline of the outlier
+kernel (algorithmic bytes = stack read once + image + mask, over its average launch time
+measured with HIP events on the launch stream) and the CPU oracle timed on the host on a
+bounded band of output rows.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [os.path.join(ROOT, "chrono-photo_amd")]
+
+HBM_PEAK_GBS = 8000.0
+
+
+class DevArray:
+    """Exposes a raw device pointer to torch through __cuda_array_interface__."""
+
+    def __init__(self, ptr, shape, typestr):
+        self.__cuda_array_interface__ = {"shape": tuple(shape), "typestr": typestr, "data": (ptr, False),
+                                         "version": 2, "strides": None}
+
+
+WORKLOADS = {
+    "outlier_4k_rgb16_256f": dict(
+        scene=dict(seed=2, width=3840, height=2160, channels=3, bytes_per_sample=2, n_frames=256,
+                   base_lo=8000, base_hi=50000, sigma=200.0, n_disks=3, disk_r_min=20, disk_r_max=40,
+                   disk_val_lo=2000, disk_val_hi=4000, shake_max=8, texture_amp=3000),
+        shake=dict(anchors=[(480, 270), (3360, 270), (1920, 1080), (480, 1890), (3360, 1890)], r=32, s=12),
+        mode="outlier", threshold=("abs", 0.05, 0.2), background="first", outlier="forward"),
+    "outlier_1080p_rgb8_64f": dict(
+        scene=dict(seed=1, width=1920, height=1080, channels=3, bytes_per_sample=1, n_frames=64,
+                   base_lo=40, base_hi=200, sigma=3.0, n_disks=3, disk_r_min=10, disk_r_max=40,
+                   disk_val_lo=15, disk_val_hi=25, blob_permille=100, blob_r=30, blob_val=230),
+        shake=None, mode="outlier", threshold=("rel", 2.0, 4.0), background="first", outlier="average"),
+    "darker_1080p_rgb8_64f": dict(
+        scene=dict(seed=1, width=1920, height=1080, channels=3, bytes_per_sample=1, n_frames=64,
+                   base_lo=40, base_hi=200, sigma=3.0, n_disks=3, disk_r_min=10, disk_r_max=40,
+                   disk_val_lo=15, disk_val_hi=25, blob_permille=100, blob_r=30, blob_val=230),
+        shake=None, mode="darker"),
+    "outlier_640x480_rgb8_16f": dict(
+        scene=dict(seed=0, width=640, height=480, channels=3, bytes_per_sample=1, n_frames=16,
+                   base_lo=40, base_hi=200, sigma=3.0, n_disks=1, disk_r_min=20, disk_r_max=20,
+                   disk_val_lo=15, disk_val_hi=25),
+        shake=None, mode="outlier", threshold=("abs", 0.05, 0.2), background="first", outlier="extreme"),
+}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="outlier_4k_rgb16_256f", choices=sorted(WORKLOADS))
+    ap.add_argument("--cpu-rows", type=int, default=256, help="output rows of the CPU baseline band")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--force-generic", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build composites on a single MI355X and has no "
+              f"multi-GPU path (asked for --gpus {args.gpus}).", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import chrono_photo_amd as cpa
+
+    wl = WORKLOADS[args.workload]
+    scene = cpa.Scene(**wl["scene"])
+    n, w, h, ch, bps = scene.n_frames, scene.width, scene.height, scene.channels, scene.bytes_per_sample
+    dt_s = "<u2" if bps == 2 else "|u1"
+    ctx = cpa.Context(0)
+    stream = torch.cuda.ExternalStream(ctx.stream)
+    t0 = time.time()
+    stack = cpa.Stack(ctx, w, h, ch, bps, capacity=n)
+    scene.generate_device(stack.device_ptr, 0, n, stream=ctx.stream)
+    ctx.synchronize()
+    lib = ctx.lib
+    cpa._check(lib.cp_stack_commit(stack.handle, n))
+    gen_s = time.time() - t0
+    print(f"[bench] {args.workload}: generated {n} frames {w}x{h}x{ch} u{8 * bps} in HBM in {gen_s:.1f}s",
+          file=sys.stderr, flush=True)
+
+    extra = {}
+    crop = None
+    if wl.get("shake"):
+        sh = wl["shake"]
+        t0 = time.time()
+        offsets = cpa.ShakeAnalyzer().analyze(stack, sh["anchors"], sh["r"], sh["s"])
+        shake_s = time.time() - t0
+        truth = scene.shake_truth()
+        crop = cpa.Crop.create(offsets, w, h)
+        terms = len(sh["anchors"]) * (2 * sh["s"] + 1) ** 2 * (2 * sh["r"] + 1) ** 2 * ch * (n - 1)
+        extra["shake"] = {"seconds": round(shake_s, 4), "sq_diff_terms_per_s": terms / shake_s,
+                          "offsets_match_seed": bool(np.array_equal(offsets, truth)),
+                          "anchors": len(sh["anchors"]), "radius": sh["r"], "search": sh["s"]}
+        print(f"[bench] shake analysis {shake_s:.3f}s, offsets match seed: "
+              f"{extra['shake']['offsets_match_seed']}", file=sys.stderr, flush=True)
+    sel = cpa._FrameSel(stack, None, crop, n)
+    oh, ow, _ = sel.out_shape(stack)
+
+    out_px = torch.empty(oh * ow * ch * bps, dtype=torch.uint8, device="cuda")
+    out_mask = torch.empty(oh * ow * ch, dtype=torch.uint8, device="cuda")
+    flags = cpa.CP_FLAG_DEVICE_OUT | (cpa.CP_FLAG_FORCE_GENERIC if args.force_generic else 0)
+    if wl["mode"] == "outlier":
+        kind, lo, hi = wl["threshold"]
+        thr = cpa.Threshold(kind == "abs", lo, hi)
+        proc = cpa.OutlierProcessor(thr, cpa.BackgroundMode.from_str(wl["background"]),
+                                    cpa.OutlierSelectionMode.from_str(wl["outlier"]))
+        params = proc.params(bps)
+        outs = cpa.cp_outputs(out_px.data_ptr(), out_mask.data_ptr(), None, None)
+
+        def step():
+            cpa._check(lib.cp_outlier(ctx.handle, stack.handle, C.byref(sel.c), C.byref(params),
+                                      C.byref(outs), flags))
+        out_bytes = oh * ow * ch * bps + oh * ow * ch
+    else:
+        weights = np.ones(4, np.float32)
+        fade = cpa.Fade.none().native()
+
+        def step():
+            cpa._check(lib.cp_simple(ctx.handle, stack.handle, C.byref(sel.c),
+                                     weights.ctypes.data_as(C.POINTER(C.c_float)), C.byref(fade),
+                                     int(wl["mode"] == "darker"), C.c_void_p(out_px.data_ptr()), flags))
+        out_bytes = oh * ow * ch * bps
+
+    torch.cuda.synchronize()
+    for _ in range(args.warmup):
+        step()
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    for _ in range(args.steps):
+        step()
+    ev1.record(stream)
+    ev1.synchronize()
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1) / args.steps
+    pixel_samples = n * oh * ow
+    value = pixel_samples / (ms / 1e3)
+    alg_bytes = n * oh * ow * ch * bps + out_bytes
+    achieved_gbs = alg_bytes / (ms / 1e3) / 1e9
+    print(f"[bench] {ms:.3f} ms/step, {value:.4g} pixel-samples/s, {achieved_gbs:.0f} GB/s "
+          f"({100 * achieved_gbs / HBM_PEAK_GBS:.1f}% of HBM peak)", file=sys.stderr, flush=True)
+
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle_py as O
+        rows = min(args.cpu_rows, oh)
+        # band of the stack that covers output rows [0, rows) under every frame's crop
+        band_h = rows + (int(crop[0][:, 1].max()) if crop is not None else 0)
+        dev = torch.as_tensor(DevArray(stack.device_ptr, (n, h, w, ch), dt_s), device="cuda")
+        band = dev[:, :band_h].cpu().numpy()
+        threads = os.cpu_count() or 1
+        if wl["mode"] == "outlier":
+            othr = O.threshold(thr.absolute, thr.min, thr.max, bps)
+            bgm, selm = int(proc.background), int(proc.outlier)
+            kw = dict(crop_xy=None if crop is None else crop[0], out_w=ow, out_h=rows)
+            t0 = time.time()
+            ref_img, _, _ = O.outlier(band, othr, bgm, selm, threads=threads, y0=0, y1=rows, **kw)
+            cpu_s = time.time() - t0
+            one_rows = max(1, rows // 16)
+            t0 = time.time()
+            O.outlier(band, othr, bgm, selm, threads=1, y0=0, y1=one_rows, **kw)
+            one_s = time.time() - t0
+        else:
+            kw = dict(out_w=ow, out_h=rows)
+            t0 = time.time()
+            ref_img = O.simple(band, darker=wl["mode"] == "darker", threads=threads, y0=0, y1=rows, **kw)
+            cpu_s = time.time() - t0
+            one_rows = max(1, rows // 16)
+            t0 = time.time()
+            O.simple(band, darker=wl["mode"] == "darker", threads=1, y0=0, y1=one_rows, **kw)
+            one_s = time.time() - t0
+        gpu_img = out_px.view(torch.uint16 if bps == 2 else torch.uint8).view(oh, ow, ch)[:rows].cpu().numpy()
+        match = bool(np.array_equal(gpu_img, ref_img[:rows]))
+        cpu = {"value": n * rows * ow / cpu_s, "unit": "pixel-samples/s", "cores": threads, "kind": "port",
+               "sample": f"output rows [0,{rows}) x {ow} px x {n} frames (CPU oracle, OpenMP row-parallel)",
+               "single_thread_value": n * one_rows * ow / one_s, "gpu_rows_bit_exact": match}
+        print(f"[bench] cpu oracle {cpu['value']:.4g} px-samples/s on {threads} threads, "
+              f"{cpu['single_thread_value']:.4g} on 1 thread; GPU rows bit-exact: {match}", file=sys.stderr)
+
+    res = {
+        "metric": "pixel-samples/s outlier mode (% HBM roofline)" if wl["mode"] == "outlier"
+        else f"pixel-samples/s {wl['mode']} mode (% HBM roofline)",
+        "value": value, "unit": "pixel-samples/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u16" if bps == 2 else "u8", "data": "synthetic",
+        "config": {"workload": args.workload, "frames": n, "width": w, "height": h, "channels": ch,
+                   "out_width": ow, "out_height": oh, "mode": wl["mode"],
+                   **({"threshold": "/".join(map(str, wl["threshold"])), "background": wl["background"],
+                       "outlier": wl["outlier"]} if wl["mode"] == "outlier" else {}),
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes_per_launch": alg_bytes},
+        "cpu_baseline": cpu,
+        **extra,
+    }
+    print(json.dumps(res))
+    stack.close()
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

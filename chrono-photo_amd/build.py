@@ -26,7 +26,8 @@ FLAGS = [
 ]
 
 LIBS = {
-    "libchrono_hip.so": ["abi.hip", "kernels.hip", "shake.hip", "batch.hip"],
+    "libchrono_hip.so": ["abi.hip", "kernels.hip", "kernels_fast_u8.hip", "kernels_fast_u16.hip",
+                         "shake.hip", "batch.hip"],
     "libchrono_synth.so": ["synth.hip"],
 }
 

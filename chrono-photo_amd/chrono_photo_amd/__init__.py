@@ -23,6 +23,7 @@ from __future__ import annotations
 import ctypes as C
 import enum
 import os
+import weakref
 from dataclasses import dataclass
 from typing import Optional, Sequence
 
@@ -122,6 +123,7 @@ class cs_scene(C.Structure):
 ABI_SYMBOLS = [
     "cp_last_error", "cp_version", "cp_ctx_create", "cp_ctx_destroy", "cp_ctx_synchronize",
     "cp_ctx_stream", "cp_stack_create", "cp_stack_wrap", "cp_stack_upload", "cp_stack_push",
+    "cp_stack_commit",
     "cp_stack_info_get", "cp_stack_destroy", "cp_threshold_new", "cp_threshold_blend_value",
     "cp_fade_table", "cp_fade_get", "cp_outlier", "cp_simple", "cp_shake", "cp_crop_create",
     "cp_video_windows", "cp_batch_run", "cp_server_create", "cp_server_submit", "cp_server_poll",
@@ -160,6 +162,7 @@ def load_library() -> C.CDLL:
                                     C.c_uint32, C.c_uint32, P(C.c_void_p)]),
         "cp_stack_upload": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32]),
         "cp_stack_push": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, P(C.c_uint32)]),
+        "cp_stack_commit": (C.c_int, [C.c_void_p, C.c_uint32]),
         "cp_stack_info_get": (C.c_int, [C.c_void_p, P(cp_stack_info)]),
         "cp_stack_destroy": (C.c_int, [C.c_void_p]),
         "cp_threshold_new": (C.c_int, [C.c_int32, C.c_float, C.c_float, C.c_uint32, P(cp_threshold)]),
@@ -430,6 +433,8 @@ class Context:
     def __init__(self, device: int = 0, hbm_budget: int = 0):
         self.lib = load_library()
         h = C.c_void_p()
+        self.handle = None
+        self._children = weakref.WeakSet()
         _check(self.lib.cp_ctx_create(device, hbm_budget, C.byref(h)))
         self.handle = h
 
@@ -441,7 +446,10 @@ class Context:
         return self.lib.cp_ctx_stream(self.handle)
 
     def close(self):
+        """Destroys the context; stacks and servers created on it are closed first."""
         if self.handle:
+            for child in list(self._children):
+                child.close()
             self.lib.cp_ctx_destroy(self.handle)
             self.handle = None
 
@@ -459,6 +467,7 @@ class Stack:
                  capacity: int = 1, device_ptr: Optional[int] = None, n_frames: Optional[int] = None):
         self.ctx = ctx
         self.lib = ctx.lib
+        self.handle = None
         h = C.c_void_p()
         if device_ptr is None:
             _check(self.lib.cp_stack_create(ctx.handle, width, height, channels, bytes_per_sample,
@@ -467,6 +476,7 @@ class Stack:
             _check(self.lib.cp_stack_wrap(ctx.handle, C.c_void_p(device_ptr), width, height, channels,
                                           bytes_per_sample, n_frames, C.byref(h)))
         self.handle = h
+        ctx._children.add(self)
         self.width, self.height, self.channels, self.bps = width, height, channels, bytes_per_sample
 
     @classmethod
@@ -497,6 +507,10 @@ class Stack:
                                       C.byref(first)))
         return first.value
 
+    def commit(self, count: int):
+        """Declares slots [0, count) filled by a device-side writer."""
+        _check(self.lib.cp_stack_commit(self.handle, count))
+
     def info(self) -> cp_stack_info:
         i = cp_stack_info()
         _check(self.lib.cp_stack_info_get(self.handle, C.byref(i)))
@@ -511,9 +525,9 @@ class Stack:
         return self.info().device_ptr
 
     def close(self):
-        if self.handle:
+        if self.handle and self.ctx.handle:
             self.lib.cp_stack_destroy(self.handle)
-            self.handle = None
+        self.handle = None
 
     def __del__(self):
         try:

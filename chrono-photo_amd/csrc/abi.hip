@@ -275,6 +275,13 @@ CP_API int cp_stack_push(cp_stack* st, const void* host_frames, uint32_t n, uint
     return CP_OK;
 }
 
+CP_API int cp_stack_commit(cp_stack* st, uint32_t count) {
+    if (!st) return fail(CP_ERR_INVALID_ARG, "null stack");
+    if (count > st->capacity) return fail(CP_ERR_CAPACITY, "count %u exceeds capacity %u", count, st->capacity);
+    st->count = count;
+    return CP_OK;
+}
+
 CP_API int cp_stack_info_get(const cp_stack* st, cp_stack_info* out) {
     if (!st || !out) return fail(CP_ERR_INVALID_ARG, "null argument");
     out->width = st->width;

@@ -1,6 +1,7 @@
 // kernels.hip -- kernel instantiation and launch policy for the compositing modes.
 #include "k_outlier_generic.h"
 #include "k_simple.h"
+#include "kernels_fast.h"
 #include "runtime.h"
 
 namespace cp {
@@ -20,9 +21,18 @@ static int launch_outlier_generic_b(const cp_stack* st, const OutlierK& k, hipSt
 int launch_outlier(cp_ctx* ctx, const cp_stack* st, const OutlierK& k, uint32_t flags,
                    hipStream_t s) {
     (void)ctx;
-    (void)flags;
-    if (st->bps == 1) launch_outlier_generic_b<1>(st, k, s);
-    else launch_outlier_generic_b<2>(st, k, s);
+    // The register-resident kernel needs the full frame set as the median sample and exact
+    // integer sums for the average modes (N * max sample < 2^24).
+    const bool sums = k.background == 2 || k.outlier == 3;
+    const bool exact_sums = st->bps == 1 || k.n <= 256;
+    const bool fast_ok = !(flags & CP_FLAG_FORCE_GENERIC) && k.sample_mask == nullptr &&
+                         (!sums || exact_sums);
+    bool done = false;
+    if (fast_ok) done = st->bps == 1 ? launch_outlier_fast_u8(st, k, s) : launch_outlier_fast_u16(st, k, s);
+    if (!done) {
+        if (st->bps == 1) launch_outlier_generic_b<1>(st, k, s);
+        else launch_outlier_generic_b<2>(st, k, s);
+    }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(CP_ERR_HIP, "outlier kernel launch: %s", hipGetErrorString(e));
     return CP_OK;

@@ -138,6 +138,9 @@ CP_API int cp_stack_wrap(cp_ctx* ctx, void* device_ptr, uint32_t width, uint32_t
 CP_API int cp_stack_upload(cp_stack* st, uint32_t slot, const void* host_frames, uint32_t n);
 /* Append n host frames; returns the first slot used (layout check as src/slicer.rs:145-157). */
 CP_API int cp_stack_push(cp_stack* st, const void* host_frames, uint32_t n, uint32_t* first_slot);
+/* Declare slots [0, count) filled by a device-side writer (a decoder or generator writing
+ * through cp_stack_info.device_ptr on the context stream). */
+CP_API int cp_stack_commit(cp_stack* st, uint32_t count);
 CP_API int cp_stack_info_get(const cp_stack* st, cp_stack_info* out);
 CP_API int cp_stack_destroy(cp_stack* st);
 
