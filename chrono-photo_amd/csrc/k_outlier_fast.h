@@ -27,6 +27,8 @@
 // pixel finishes: background and outlier selection / blending (src/chrono.rs:290-493),
 // re-reading from L2 only the few outlier samples an order-dependent blend needs.
 #pragma once
+#include <type_traits>
+
 #include "device_common.h"
 #include "k_outlier_generic.h"
 
@@ -37,79 +39,89 @@ typedef unsigned short us2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ us2 as_us2(uint32_t v) { return __builtin_bit_cast(us2, v); }
 __device__ __forceinline__ uint32_t as_u32(us2 v) { return __builtin_bit_cast(uint32_t, v); }
 
-// Sum / min / max over the G adjacent lanes of a pixel group (G in {1,2,4,8}).
+// Sum / min / max over the G adjacent lanes of a pixel group (G in {1,2,4,8}); the DPP
+// moves fold into v_add/v_min/v_max_u32_dpp.
 template <int G>
 __device__ __forceinline__ int32_t grp_sum(int32_t v) {
-    if constexpr (G >= 2) v += __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, false);   // xor 1
-    if constexpr (G >= 4) v += __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, false);   // xor 2
-    if constexpr (G >= 8) v += __builtin_amdgcn_mov_dpp(v, 0x141, 0xF, 0xF, false);  // half-row mirror
+    if constexpr (G >= 2) v += __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, true);   // xor 1
+    if constexpr (G >= 4) v += __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, true);   // xor 2
+    if constexpr (G >= 8) v += __builtin_amdgcn_update_dpp(0, v, 0x141, 0xF, 0xF, true);  // half-row mirror
     return v;
 }
 
 template <int G>
 __device__ __forceinline__ uint32_t grp_min_u32(uint32_t v) {
-    if constexpr (G >= 2) v = min(v, (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, false));
-    if constexpr (G >= 4) v = min(v, (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x4E, 0xF, 0xF, false));
-    if constexpr (G >= 8) v = min(v, (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x141, 0xF, 0xF, false));
+    if constexpr (G >= 2) v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)v, 0xB1, 0xF, 0xF, false));
+    if constexpr (G >= 4) v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)v, 0x4E, 0xF, 0xF, false));
+    if constexpr (G >= 8) v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)v, 0x141, 0xF, 0xF, false));
     return v;
 }
 
 template <int G>
 __device__ __forceinline__ uint32_t grp_max_u32(uint32_t v) {
-    if constexpr (G >= 2) v = max(v, (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, false));
-    if constexpr (G >= 4) v = max(v, (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x4E, 0xF, 0xF, false));
-    if constexpr (G >= 8) v = max(v, (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x141, 0xF, 0xF, false));
+    if constexpr (G >= 2) v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false));
+    if constexpr (G >= 4) v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, false));
+    if constexpr (G >= 8) v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x141, 0xF, 0xF, false));
     return v;
 }
 
-// Largest v in [0, 255] with #{bytes < v} <= k over the group's ND dwords per lane.
+// #{bytes < v} over the group, v in [1, 255].
+template <int G, int ND>
+__device__ __forceinline__ uint32_t byte_count_lt(const uint32_t (&y)[ND], uint32_t v, int32_t ntot) {
+    const uint32_t V = v * 0x01010101u, Vm = V - 0x01010101u;
+    uint32_t a = 0, b = 0;
+#pragma unroll
+    for (int i = 0; i < ND; ++i) {
+        a = __builtin_amdgcn_sad_u8(y[i], V, a);
+        b = __builtin_amdgcn_sad_u8(y[i], Vm, b);
+    }
+    return (uint32_t)(ntot + grp_sum<G>((int32_t)(a - b))) >> 1;
+}
+
+// Largest v in [0, 255] with #{bytes < v} <= k over the group's ND dwords per lane.  The
+// candidate is kept replicated in all four bytes, so each step is two v_sad_u8 per dword,
+// a G-lane reduction and a select.
 template <int G, int ND>
 __device__ __forceinline__ uint32_t byte_select(const uint32_t (&y)[ND], uint32_t k, int32_t ntot) {
     uint32_t lo = 0;
+    const int32_t two_k = 2 * (int32_t)k;
 #pragma unroll 1
     for (int bit = 7; bit >= 0; --bit) {
-        const uint32_t v = lo | (1u << bit);
-        const uint32_t V = v * 0x01010101u, Vm = V - 0x01010101u;
+        const uint32_t V = lo | (0x01010101u << bit);
+        const uint32_t Vm = V - 0x01010101u;
         uint32_t a = 0, b = 0;
 #pragma unroll
         for (int i = 0; i < ND; ++i) {
             a = __builtin_amdgcn_sad_u8(y[i], V, a);
             b = __builtin_amdgcn_sad_u8(y[i], Vm, b);
         }
-        const int32_t d = grp_sum<G>((int32_t)(a - b));
-        const uint32_t cnt = (uint32_t)(ntot + d) >> 1;
-        if (cnt <= k) lo = v;
+        // ntot + d = 2 #{y < v}
+        if (ntot + grp_sum<G>((int32_t)(a - b)) <= two_k) lo = V;
     }
-    return lo;
+    return lo & 0xFFu;
 }
 
-// r-th smallest sample of one channel (raw: NW dwords of u16 halves per lane).
-template <int B, int G, int NW>
-__device__ __forceinline__ uint32_t select_rank(const uint32_t (&r)[NW], uint32_t k, int32_t ntot) {
+// Exact r-th smallest sample of one 16-bit channel by two byte levels.
+template <int G, int NW>
+__device__ __forceinline__ uint32_t select_rank_u16(const uint32_t (&r)[NW], uint32_t k, int32_t ntot) {
     constexpr int ND = NW / 2;
     uint32_t y[ND];
-    if constexpr (B == 1) {
 #pragma unroll
-        for (int i = 0; i < ND; ++i) y[i] = __builtin_amdgcn_perm(r[2 * i + 1], r[2 * i], 0x06040200u);
-        return byte_select<G, ND>(y, k, ntot);
-    } else {
+    for (int i = 0; i < ND; ++i) y[i] = __builtin_amdgcn_perm(r[2 * i + 1], r[2 * i], 0x07050301u);
+    const uint32_t H = byte_select<G, ND>(y, k, ntot);
+    const us2 base = {(unsigned short)(H << 8), (unsigned short)(H << 8)};
+    const us2 c255 = {255, 255};
 #pragma unroll
-        for (int i = 0; i < ND; ++i) y[i] = __builtin_amdgcn_perm(r[2 * i + 1], r[2 * i], 0x07050301u);
-        const uint32_t H = byte_select<G, ND>(y, k, ntot);
-        const us2 base = {(unsigned short)(H << 8), (unsigned short)(H << 8)};
-        const us2 c255 = {255, 255};
-#pragma unroll
-        for (int i = 0; i < ND; ++i) {
-            us2 a = __builtin_elementwise_min(__builtin_elementwise_sub_sat(as_us2(r[2 * i]), base), c255);
-            us2 b = __builtin_elementwise_min(__builtin_elementwise_sub_sat(as_us2(r[2 * i + 1]), base), c255);
-            y[i] = __builtin_amdgcn_perm(as_u32(b), as_u32(a), 0x06040200u);
-        }
-        const uint32_t L = byte_select<G, ND>(y, k, ntot);
-        return (H << 8) | L;
+    for (int i = 0; i < ND; ++i) {
+        us2 a = __builtin_elementwise_min(__builtin_elementwise_sub_sat(as_us2(r[2 * i]), base), c255);
+        us2 b = __builtin_elementwise_min(__builtin_elementwise_sub_sat(as_us2(r[2 * i + 1]), base), c255);
+        y[i] = __builtin_amdgcn_perm(as_u32(b), as_u32(a), 0x06040200u);
     }
+    const uint32_t L = byte_select<G, ND>(y, k, ntot);
+    return (H << 8) | L;
 }
 
-// #{x <= v} over the group (v < maxv).
+// #{x <= v} over the group (v below the maximum sample value).
 template <int G, int NW>
 __device__ __forceinline__ uint32_t count_le(const uint32_t (&r)[NW], uint32_t v, int32_t ntot) {
     const uint32_t V0 = v | (v << 16), V1 = (v + 1) | ((v + 1) << 16);
@@ -134,14 +146,83 @@ __device__ __forceinline__ uint32_t min_greater(const uint32_t (&r)[NW], uint32_
     return grp_min_u32<G>(mm) + v + 1;
 }
 
-// Values at ranks (r0, r1) with r1 in {r0, r0 + 1}.
-template <int B, int G, int NW>
+__device__ __forceinline__ us2 med3_us2(us2 a, us2 b, us2 c) {
+    const us2 lo = __builtin_elementwise_min(a, b), hi = __builtin_elementwise_max(a, b);
+    return __builtin_elementwise_max(lo, __builtin_elementwise_min(hi, c));
+}
+
+// Location estimate of a 16-bit channel: median-of-3 "remedian" over the lane's samples
+// (both halves of each dword at once), then a trimmed mean over the group's lanes.  It only
+// places the search window; exactness never depends on it.
+template <int G, int NW>
+__device__ __forceinline__ uint32_t group_estimate(const uint32_t (&r)[NW]) {
+    constexpr int L0 = NW / 3, L1 = L0 / 3, L2 = L1 / 3;
+    static_assert(L0 >= 1, "estimate needs at least three dwords per lane");
+    us2 a[L0];
+#pragma unroll
+    for (int i = 0; i < L0; ++i) a[i] = med3_us2(as_us2(r[3 * i]), as_us2(r[3 * i + 1]), as_us2(r[3 * i + 2]));
+    if constexpr (L1 >= 1) {
+#pragma unroll
+        for (int i = 0; i < L1; ++i) a[i] = med3_us2(a[3 * i], a[3 * i + 1], a[3 * i + 2]);
+    }
+    if constexpr (L2 >= 1) {
+#pragma unroll
+        for (int i = 0; i < L2; ++i) a[i] = med3_us2(a[3 * i], a[3 * i + 1], a[3 * i + 2]);
+    }
+    const uint32_t e = ((uint32_t)a[0].x + (uint32_t)a[0].y) >> 1;
+    if constexpr (G == 1) return e;
+    const uint32_t s = (uint32_t)grp_sum<G>((int32_t)e);
+    if constexpr (G == 2) return s >> 1;
+    return (s - grp_min_u32<G>(e) - grp_max_u32<G>(e)) / (uint32_t)(G - 2);
+}
+
+// Values at ranks (r0, r1), r1 in {r0, r0 + 1}, of one channel.
+//  u8:  one byte level; the neighbour comes from one more byte count.
+//  u16 (median): an 8-step byte search inside the window [lo, lo + 255] around the
+//       estimate; the result is exact when it ends strictly inside (1..254), otherwise
+//       the exact two-level search runs.  Group-uniform branches only.
+template <int B, int G, int NW, bool WINDOW>
 __device__ __forceinline__ void rank_pair(const uint32_t (&r)[NW], uint32_t r0, uint32_t r1, int32_t ntot,
                                           uint32_t& v0, uint32_t& v1) {
+    constexpr int ND = NW / 2;
     constexpr uint32_t MAXV = B == 1 ? 255u : 65535u;
-    v0 = select_rank<B, G, NW>(r, r0, ntot);
+    uint32_t y[ND];
+    uint32_t cle = 0;
+    bool have_cle = false;
+    if constexpr (B == 1) {
+#pragma unroll
+        for (int i = 0; i < ND; ++i) y[i] = __builtin_amdgcn_perm(r[2 * i + 1], r[2 * i], 0x06040200u);
+        v0 = byte_select<G, ND>(y, r0, ntot);
+        if (r1 != r0) {
+            cle = v0 >= 255u ? (uint32_t)ntot : byte_count_lt<G, ND>(y, v0 + 1, ntot);
+            have_cle = true;
+        }
+    } else if constexpr (WINDOW) {
+        const uint32_t est = group_estimate<G, NW>(r);
+        const uint32_t lo = min(est >= 127u ? est - 127u : 0u, 65535u - 255u);
+        const us2 base = {(unsigned short)lo, (unsigned short)lo};
+        const us2 c255 = {255, 255};
+#pragma unroll
+        for (int i = 0; i < ND; ++i) {
+            us2 a = __builtin_elementwise_min(__builtin_elementwise_sub_sat(as_us2(r[2 * i]), base), c255);
+            us2 b = __builtin_elementwise_min(__builtin_elementwise_sub_sat(as_us2(r[2 * i + 1]), base), c255);
+            y[i] = __builtin_amdgcn_perm(as_u32(b), as_u32(a), 0x06040200u);
+        }
+        const uint32_t ys = byte_select<G, ND>(y, r0, ntot);
+        if (ys >= 1u && ys <= 254u) {
+            v0 = lo + ys;
+            if (r1 != r0) {
+                cle = byte_count_lt<G, ND>(y, ys + 1, ntot);  // #{x < v0 + 1}
+                have_cle = true;
+            }
+        } else {
+            v0 = select_rank_u16<G, NW>(r, r0, ntot);
+        }
+    } else {
+        v0 = select_rank_u16<G, NW>(r, r0, ntot);
+    }
     if (r1 == r0) { v1 = v0; return; }
-    const uint32_t cle = v0 >= MAXV ? (uint32_t)ntot : count_le<G, NW>(r, v0, ntot);
+    if (!have_cle) cle = v0 >= MAXV ? (uint32_t)ntot : count_le<G, NW>(r, v0, ntot);
     v1 = cle >= r1 + 1 ? v0 : min_greater<G, NW>(r, v0);
 }
 
@@ -235,11 +316,11 @@ __global__ void __launch_bounds__(256) outlier_fast_kernel(OutlierK k) {
         if (k.w[c] == 0.0f) continue;
         uint32_t vals[6] = {0, 0, 0, 0, 0, 0};
         uint32_t m0, m1;
-        rank_pair<B, G, NW>(raw[c], plan.m0, plan.m1, ntot, m0, m1);
+        rank_pair<B, G, NW, ABS>(raw[c], plan.m0, plan.m1, ntot, m0, m1);
         uint32_t a0 = 0, a1 = 0, b0 = 0, b1 = 0;
         if constexpr (!ABS) {
-            rank_pair<B, G, NW>(raw[c], plan.a0, plan.a1, ntot, a0, a1);
-            rank_pair<B, G, NW>(raw[c], plan.b0, plan.b1, ntot, b0, b1);
+            rank_pair<B, G, NW, false>(raw[c], plan.a0, plan.a1, ntot, a0, a1);
+            rank_pair<B, G, NW, false>(raw[c], plan.b0, plan.b1, ntot, b0, b1);
         }
 #pragma unroll
         for (int i = 0; i < 6; ++i) {
@@ -258,39 +339,105 @@ __global__ void __launch_bounds__(256) outlier_fast_kernel(OutlierK k) {
 
     // ---- classification over the lane's frames
     float wi[4], sg[4];
+    bool unit = ABS, pos = true;
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
         wi[c] = ABS ? k.w[c] : k.w[c] * iqr_inv[c];
         sg[c] = signum_f(k.w[c]);
+        if (c < C) {
+            unit = unit && k.w[c] == 1.0f;
+            pos = pos && !(sg[c] < 0.0f);
+        }
     }
     const float tsq = k.th.min * k.th.min;
     const bool want_max = k.outlier == 2 || k.stats != nullptr;
     const bool want_sums = k.background == 2 || k.outlier == 3;
+    const bool full_tile = N == (uint32_t)(G * F);
     uint32_t m_lo = 0, m_hi = 0;
     float lmax = 0.0f;
     uint32_t lmaxi = NONE;
     uint32_t sum_all[4] = {0, 0, 0, 0}, sum_out[4] = {0, 0, 0, 0};
+
+    // Bound check: with non-negative weights every frame's f32 distance is at most the
+    // same f32 expression on the per-channel maximum deviation from the median (correct
+    // rounding is monotone), so when that bound is below the threshold no frame of this
+    // lane can be an outlier and the per-frame pass is skipped.
+    bool may_outlier = true;
+    if (pos && full_tile) {
+        float bound = 0.0f;
 #pragma unroll
-    for (int j = 0; j < F; ++j) {
-        const uint32_t t = t0 + j;
-        uint32_t s[4] = {0, 0, 0, 0};
+        for (int c = 0; c < C; ++c) {
+            us2 mn = {0xFFFF, 0xFFFF}, mx = {0, 0};
 #pragma unroll
-        for (int c = 0; c < C; ++c) s[c] = (j & 1) ? (raw[c][j >> 1] >> 16) : (raw[c][j >> 1] & 0xFFFFu);
-        const float d = frame_dist<C>(s, med, wi, sg);
-        const bool o = d >= tsq;  // pads are masked out after the loop
-        if (j < 32) m_lo |= (uint32_t)o << j;
-        else m_hi |= (uint32_t)o << (j - 32);
-        if (want_max) {
-            const bool valid = t < N;
-            if (o && valid && d > lmax) { lmax = d; lmaxi = t; }
+            for (int i = 0; i < NW; ++i) {
+                mn = __builtin_elementwise_min(mn, as_us2(raw[c][i]));
+                mx = __builtin_elementwise_max(mx, as_us2(raw[c][i]));
+            }
+            const float lo = (float)min((uint32_t)mn.x, (uint32_t)mn.y);
+            const float hi = (float)max((uint32_t)mx.x, (uint32_t)mx.y);
+            const float dev = fmaxf(med[c] - lo, hi - med[c]);
+            const float a = unit ? dev : wi[c] * dev;
+            bound = c == 0 ? a * a : bound + a * a;
         }
-        if (want_sums) {
-            const bool valid = t < N;
+        may_outlier = bound >= tsq;
+    }
+
+    auto classify = [&](auto unit_t, auto pos_t, auto max_t, auto sums_t) {
+        constexpr bool UNIT = decltype(unit_t)::value, POS = decltype(pos_t)::value;
+        constexpr bool MAXT = decltype(max_t)::value, SUMS = decltype(sums_t)::value;
+#pragma unroll
+        for (int j = 0; j < F; ++j) {
+            uint32_t xs[4];
+            float d = 0.0f;
 #pragma unroll
             for (int c = 0; c < C; ++c) {
-                sum_all[c] += valid ? s[c] : 0u;
-                sum_out[c] += (o && valid) ? s[c] : 0u;
+                const uint32_t w32 = raw[c][j >> 1];
+                xs[c] = (j & 1) ? (w32 >> 16) : (w32 & 0xFFFFu);
+                const float diff = med[c] - (float)xs[c];
+                const float a = UNIT ? diff : wi[c] * diff;
+                float term = a * a;
+                if constexpr (!POS) term = sg[c] * term;
+                d = c == 0 ? term : d + term;  // 0 + term0 == term0 up to the sign of zero
             }
+            const uint32_t o = d >= tsq ? 1u : 0u;  // pads are masked out after the loop
+            if (j < 32) m_lo |= o << j;
+            else m_hi |= o << (j - 32);
+            if constexpr (MAXT || SUMS) {
+                const bool ov = o && (t0 + j < N);
+                if constexpr (MAXT) {
+                    if (ov && d > lmax) { lmax = d; lmaxi = t0 + j; }
+                }
+                if constexpr (SUMS) {
+#pragma unroll
+                    for (int c = 0; c < C; ++c) sum_out[c] += ov ? xs[c] : 0u;
+                }
+            }
+        }
+    };
+    using T_ = std::true_type;
+    using F_ = std::false_type;
+    if (may_outlier) {
+        if (want_max) {
+            if (want_sums) classify(F_{}, F_{}, T_{}, T_{});
+            else if (unit) classify(T_{}, T_{}, T_{}, F_{});
+            else if (pos) classify(F_{}, T_{}, T_{}, F_{});
+            else classify(F_{}, F_{}, T_{}, F_{});
+        } else {
+            if (want_sums) classify(F_{}, F_{}, F_{}, T_{});
+            else if (unit) classify(T_{}, T_{}, F_{}, F_{});
+            else if (pos) classify(F_{}, T_{}, F_{}, F_{});
+            else classify(F_{}, F_{}, F_{}, F_{});
+        }
+    }
+    if (want_sums) {
+        // all-frame sums through v_sad_u16 against zero; pads (MAXV) are taken back out
+        const uint32_t npad = t0 + F > N ? min(t0 + F - N, (uint32_t)F) : 0u;
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            uint32_t acc = 0;
+#pragma unroll
+            for (int i = 0; i < NW; ++i) acc = __builtin_amdgcn_sad_u16(raw[c][i], 0u, acc);
+            sum_all[c] = acc - npad * MAXV;
         }
     }
 

@@ -12,7 +12,7 @@ inline bool fast_tile_for(uint32_t n, int* G, int* F) {
     if (n <= 32) { *G = 1; *F = 32; return true; }
     if (n <= 64) { *G = 2; *F = 32; return true; }
     if (n <= 128) { *G = 4; *F = 32; return true; }
-    if (n <= 256) { *G = 4; *F = 64; return true; }
+    if (n <= 256) { *G = 8; *F = 32; return true; }
     if (n <= 512) { *G = 8; *F = 64; return true; }
     return false;
 }
@@ -33,7 +33,7 @@ inline bool launch_fast_c(const OutlierK& k, hipStream_t s) {
     else if (G == 1 && F == 32) launch_fast_shape<B, C, 1, 32>(k, s);
     else if (G == 2) launch_fast_shape<B, C, 2, 32>(k, s);
     else if (G == 4 && F == 32) launch_fast_shape<B, C, 4, 32>(k, s);
-    else if (G == 4) launch_fast_shape<B, C, 4, 64>(k, s);
+    else if (G == 8 && F == 32) launch_fast_shape<B, C, 8, 32>(k, s);
     else launch_fast_shape<B, C, 8, 64>(k, s);
     return true;
 }
