@@ -69,7 +69,19 @@ def main():
     ap.add_argument("--cpu-rows", type=int, default=256, help="output rows of the CPU baseline band")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--force-generic", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device copy (scale factor for HBM byte counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        src = torch.empty(1 << 30, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_read": 1 << 30, "copy_calibration_bytes_written": 1 << 30,
+                          "launches": 3}))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build composites on a single MI355X and has no "
               f"multi-GPU path (asked for --gpus {args.gpus}).", file=sys.stderr)
@@ -108,6 +120,7 @@ def main():
         terms = len(sh["anchors"]) * (2 * sh["s"] + 1) ** 2 * (2 * sh["r"] + 1) ** 2 * ch * (n - 1)
         extra["shake"] = {"seconds": round(shake_s, 4), "sq_diff_terms_per_s": terms / shake_s,
                           "offsets_match_seed": bool(np.array_equal(offsets, truth)),
+                          "frames_off_seed": int((offsets != truth).any(axis=1).sum()),
                           "anchors": len(sh["anchors"]), "radius": sh["r"], "search": sh["s"]}
         print(f"[bench] shake analysis {shake_s:.3f}s, offsets match seed: "
               f"{extra['shake']['offsets_match_seed']}", file=sys.stderr, flush=True)

@@ -251,23 +251,21 @@ __device__ __forceinline__ float frame_dist(const uint32_t (&s)[4], const float 
     return d;
 }
 
+// One tile: P = 256/G pixels of row y starting at pixel bx * P.  s_off holds the element
+// offsets of the N frame positions (LDS).  Lanes that do not finish a pixel return early.
 template <int B, int C, int G, int F, bool ABS>
-__global__ void __launch_bounds__(256) outlier_fast_kernel(OutlierK k) {
+__device__ __forceinline__ void outlier_tile(const OutlierK& k, uint32_t bx, uint32_t y,
+                                             const uint64_t* __restrict__ s_off) {
     static_assert(F % 4 == 0 && F <= 64 && G <= 8, "tile shape");
     constexpr int P = 256 / G;
     constexpr int NW = F / 2;
     constexpr uint32_t MAXV = B == 1 ? 255u : 65535u;
     constexpr uint32_t NONE = 0xFFFFFFFFu;
-    __shared__ uint64_t s_off[G * F];
-
     const uint32_t N = k.n;
-    for (uint32_t i = threadIdx.x; i < N; i += blockDim.x) s_off[i] = k.pos_off[i];
-    __syncthreads();
 
     const uint32_t g = threadIdx.x % G;
     const uint32_t p = threadIdx.x / G;
-    const uint32_t x = blockIdx.x * P + p;
-    const uint32_t y = blockIdx.y;
+    const uint32_t x = bx * P + p;
     const bool px_ok = x < k.out_w;
     const uint64_t pix = ((uint64_t)y * k.stack_w + (px_ok ? x : 0u)) * C;
     const uint32_t t0 = g * F;
@@ -641,6 +639,14 @@ __global__ void __launch_bounds__(256) outlier_fast_kernel(OutlierK k) {
         st.pad = 0;
         k.stats[pix_id] = st;
     }
+}
+
+template <int B, int C, int G, int F, bool ABS>
+__global__ void __launch_bounds__(256) outlier_fast_kernel(OutlierK k) {
+    __shared__ uint64_t s_off[G * F];
+    for (uint32_t i = threadIdx.x; i < k.n; i += blockDim.x) s_off[i] = k.pos_off[i];
+    __syncthreads();
+    outlier_tile<B, C, G, F, ABS>(k, blockIdx.x, blockIdx.y, s_off);
 }
 
 }  // namespace cp

@@ -229,3 +229,33 @@ def test_shake_crop_outlier_pipeline(ctx):
     assert crop is not None and np.array_equal(crop[0], ocrop[0]) and crop[1:] == ocrop[1:]
     gpu, ref = run_both(ctx, x, True, 0.05, 0.2, 0, 4, crop=crop)
     assert_same(gpu, ref, "shake crop")
+
+
+# ------------------------------------------------------------------ synthetic scenes
+
+def test_device_generator_matches_host(ctx):
+    import torch
+    for bps in (1, 2):
+        scene = cpa.Scene(seed=9, width=200, height=150, bytes_per_sample=bps, n_frames=5,
+                          base_lo=40 if bps == 1 else 8000, base_hi=200 if bps == 1 else 50000,
+                          sigma=3.0 if bps == 1 else 200.0, n_disks=2, disk_r_min=8, disk_r_max=20,
+                          disk_val_lo=15, disk_val_hi=25, blob_permille=300, shake_max=5, texture_amp=30)
+        host = scene.generate()
+        dev = torch.zeros(host.nbytes, dtype=torch.uint8, device="cuda")
+        scene.generate_device(dev.data_ptr(), 0, 5)
+        torch.cuda.synchronize()
+        got = dev.cpu().numpy().view(host.dtype).reshape(host.shape)
+        assert np.array_equal(got, host), bps
+
+
+def test_shake_bench_radii_mid_size(ctx):
+    """Anchor radius 32, search radius 12, five anchors, as in the 4K benchmark."""
+    scene = cpa.Scene(seed=2, width=640, height=480, bytes_per_sample=2, n_frames=6, base_lo=8000,
+                      base_hi=50000, sigma=200, n_disks=0, shake_max=8, texture_amp=3000)
+    x = scene.generate()
+    st = cpa.Stack.from_array(ctx, x)
+    anchors = [(60, 60), (580, 60), (320, 240), (60, 420), (580, 420)]
+    off = cpa.ShakeAnalyzer().analyze(st, anchors, 32, 12)
+    ref = O.shake(x, anchors, 32, 12)
+    assert np.array_equal(ref, scene.shake_truth())
+    assert np.array_equal(off, ref), (off.tolist(), ref.tolist())
