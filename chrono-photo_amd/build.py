@@ -26,8 +26,8 @@ FLAGS = [
 ]
 
 LIBS = {
-    "libchrono_hip.so": ["abi.hip", "kernels.hip", "kernels_fast_u8.hip", "kernels_fast_u16.hip",
-                         "shake.hip", "batch.hip"],
+    "libchrono_hip.so": ["abi.hip", "kernels.hip", "kernels_fast_b1_c3.hip", "kernels_fast_b1_c4.hip",
+                         "kernels_fast_b2_c3.hip", "kernels_fast_b2_c4.hip", "shake.hip", "batch.hip"],
     "libchrono_synth.so": ["synth.hip"],
 }
 

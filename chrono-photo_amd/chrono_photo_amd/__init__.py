@@ -116,14 +116,14 @@ class cs_scene(C.Structure):
                 ("n_disks", C.c_uint32), ("disk_r_min", C.c_int32), ("disk_r_max", C.c_int32),
                 ("disk_val_lo", C.c_int32), ("disk_val_hi", C.c_int32), ("blob_permille", C.c_uint32),
                 ("blob_r", C.c_int32), ("blob_val", C.c_int32), ("shake_max", C.c_int32),
-                ("texture_amp", C.c_int32)]
+                ("texture_amp", C.c_int32), ("disk_y_lo", C.c_int32), ("disk_y_hi", C.c_int32)]
 
 
 # Every symbol include/chrono_hip.h declares (checked by tests/test_abi.py).
 ABI_SYMBOLS = [
     "cp_last_error", "cp_version", "cp_ctx_create", "cp_ctx_destroy", "cp_ctx_synchronize",
     "cp_ctx_stream", "cp_stack_create", "cp_stack_wrap", "cp_stack_upload", "cp_stack_push",
-    "cp_stack_commit",
+    "cp_stack_commit", "cp_stack_download",
     "cp_stack_info_get", "cp_stack_destroy", "cp_threshold_new", "cp_threshold_blend_value",
     "cp_fade_table", "cp_fade_get", "cp_outlier", "cp_simple", "cp_shake", "cp_crop_create",
     "cp_video_windows", "cp_batch_run", "cp_server_create", "cp_server_submit", "cp_server_poll",
@@ -163,6 +163,7 @@ def load_library() -> C.CDLL:
         "cp_stack_upload": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32]),
         "cp_stack_push": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, P(C.c_uint32)]),
         "cp_stack_commit": (C.c_int, [C.c_void_p, C.c_uint32]),
+        "cp_stack_download": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32]),
         "cp_stack_info_get": (C.c_int, [C.c_void_p, P(cp_stack_info)]),
         "cp_stack_destroy": (C.c_int, [C.c_void_p]),
         "cp_threshold_new": (C.c_int, [C.c_int32, C.c_float, C.c_float, C.c_uint32, P(cp_threshold)]),
@@ -507,6 +508,13 @@ class Stack:
                                       C.byref(first)))
         return first.value
 
+    def download(self, slot: int = 0, n: Optional[int] = None) -> np.ndarray:
+        """Copies frames [slot, slot + n) back to the host."""
+        n = self.count - slot if n is None else n
+        out = np.zeros((n, self.height, self.width, self.channels), self.dtype)
+        _check(self.lib.cp_stack_download(self.handle, slot, out.ctypes.data_as(C.c_void_p), n))
+        return out
+
     def commit(self, count: int):
         """Declares slots [0, count) filled by a device-side writer."""
         _check(self.lib.cp_stack_commit(self.handle, count))
@@ -692,6 +700,8 @@ class Scene:
     blob_val: int = 230
     shake_max: int = 0
     texture_amp: int = 0
+    disk_y_lo: int = 0
+    disk_y_hi: int = 0
 
     def native(self) -> cs_scene:
         s = cs_scene()

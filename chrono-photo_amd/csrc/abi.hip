@@ -6,6 +6,7 @@
 #include <cstdlib>
 
 #include "k_outlier_generic.h"
+#include "kernels_fast.h"
 #include "runtime.h"
 
 namespace cp {
@@ -138,6 +139,13 @@ int cp_ctx::ensure(cp::DeviceBuffer& b, size_t bytes) {
     return CP_OK;
 }
 
+int cp_ctx::shape_for(uint32_t bps, uint32_t channels, uint32_t n) const {
+    const int b = cp::bucket_for(n);
+    if (b < 0 || bps < 1 || bps > 2 || channels < 1 || channels > 4) return -1;
+    const int t = tuned[bps - 1][channels - 1][b];
+    return t >= 0 ? t : cp::kBucketCand[b][0];
+}
+
 int cp_ctx::charge(uint64_t bytes) {
     if (budget && used + bytes > budget)
         return cp::fail(CP_ERR_OUT_OF_HBM_BUDGET, "HBM budget %llu exceeded (%llu used, %llu asked)",
@@ -163,6 +171,7 @@ CP_API int cp_ctx_create(int32_t device, uint64_t hbm_budget_bytes, cp_ctx** out
     if (device < 0 || device >= n) return fail(CP_ERR_INVALID_ARG, "device %d out of range", device);
     CP_HIP_TRY(hipSetDevice(device));
     cp_ctx* c = new cp_ctx();
+    memset(c->tuned, -1, sizeof(c->tuned));
     c->device = device;
     c->budget = hbm_budget_bytes;
     hipDeviceProp_t prop;
@@ -272,6 +281,17 @@ CP_API int cp_stack_push(cp_stack* st, const void* host_frames, uint32_t n, uint
     int rc = cp_stack_upload(st, first, host_frames, n);
     if (rc) return rc;
     if (first_slot) *first_slot = first;
+    return CP_OK;
+}
+
+CP_API int cp_stack_download(const cp_stack* st, uint32_t slot, void* host_frames, uint32_t n) {
+    if (!st || (!host_frames && n)) return fail(CP_ERR_INVALID_ARG, "null argument");
+    if ((uint64_t)slot + n > st->count)
+        return fail(CP_ERR_INVALID_ARG, "slots [%u,%u) not loaded (stack holds %u)", slot, slot + n, st->count);
+    (void)hipSetDevice(st->ctx->device);
+    CP_HIP_TRY(hipMemcpyAsync(host_frames, st->dev + (uint64_t)slot * st->frame_bytes(),
+                              (uint64_t)n * st->frame_bytes(), hipMemcpyDeviceToHost, st->ctx->stream));
+    CP_HIP_TRY(hipStreamSynchronize(st->ctx->stream));
     return CP_OK;
 }
 

@@ -546,7 +546,12 @@ __device__ __forceinline__ void outlier_tile(const OutlierK& k, uint32_t bx, uin
                 bg_index = pos;
             }
         }
-        P_.sample(bg_index, s);
+        if (bg_index == 0) {  // frame 0 is this lane's first register half
+#pragma unroll
+            for (int c = 0; c < C; ++c) s[c] = raw[c][0] & 0xFFFFu;
+        } else {
+            P_.sample(bg_index, s);
+        }
 #pragma unroll
         for (int c = 0; c < C; ++c) pixel[c] = s[c];
     }

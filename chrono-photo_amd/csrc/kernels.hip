@@ -28,7 +28,10 @@ int launch_outlier(cp_ctx* ctx, const cp_stack* st, const OutlierK& k, uint32_t 
     const bool fast_ok = !(flags & CP_FLAG_FORCE_GENERIC) && k.sample_mask == nullptr &&
                          (!sums || exact_sums);
     bool done = false;
-    if (fast_ok) done = st->bps == 1 ? launch_outlier_fast_u8(st, k, s) : launch_outlier_fast_u16(st, k, s);
+    if (fast_ok) {
+        const int shape = ctx->shape_for(st->bps, st->channels, k.n);
+        if (shape >= 0) done = launch_outlier_fast(st->bps, st->channels, k, shape, s);
+    }
     if (!done) {
         if (st->bps == 1) launch_outlier_generic_b<1>(st, k, s);
         else launch_outlier_generic_b<2>(st, k, s);

@@ -1,20 +1,32 @@
-// kernels_fast.h -- tile-shape policy of the register-resident outlier kernel.
+// kernels_fast.h -- tile shapes of the register-resident outlier kernel and their launch.
+//
+// A shape (G, F) puts G lanes on each pixel and F frames in each lane (G * F >= N); a
+// 256-lane workgroup then covers 256 / G pixels.  Frame counts fall into buckets; each bucket
+// has up to two candidate shapes and the autotuner (cp_autotune) measures both on the device
+// and records the faster one per (sample width, channels, bucket) in the context.
 #pragma once
 #include "k_outlier_fast.h"
 #include "runtime.h"
 
 namespace cp {
 
-// Lanes per pixel (G) and frames per lane (F) for a stack of n frames: the smallest tile
-// with G * F >= n.  Returns false when n exceeds the largest tile.
-inline bool fast_tile_for(uint32_t n, int* G, int* F) {
-    if (n <= 16) { *G = 1; *F = 16; return true; }
-    if (n <= 32) { *G = 1; *F = 32; return true; }
-    if (n <= 64) { *G = 2; *F = 32; return true; }
-    if (n <= 128) { *G = 4; *F = 32; return true; }
-    if (n <= 256) { *G = 8; *F = 32; return true; }
-    if (n <= 512) { *G = 8; *F = 64; return true; }
-    return false;
+struct TileShape {
+    int G, F;
+};
+
+constexpr int kNumShapes = 10;
+constexpr TileShape kShapes[kNumShapes] = {{1, 16}, {1, 32}, {2, 16}, {2, 32}, {4, 16},
+                                           {4, 32}, {8, 16}, {8, 32}, {4, 64}, {8, 64}};
+constexpr int kNumBuckets = 6;
+constexpr uint32_t kBucketMax[kNumBuckets] = {16, 32, 64, 128, 256, 512};
+constexpr int kBucketCand[kNumBuckets][2] = {{0, -1}, {1, 2}, {3, 4}, {5, 6}, {7, 8}, {9, -1}};
+// Shapes the persistent batched kernel carries (F <= 32 keeps its register budget).
+constexpr int kBatchShapeMax = 7;
+
+inline int bucket_for(uint32_t n) {
+    for (int b = 0; b < kNumBuckets; ++b)
+        if (n <= kBucketMax[b]) return b;
+    return -1;
 }
 
 template <int B, int C, int G, int F>
@@ -26,19 +38,34 @@ inline void launch_fast_shape(const OutlierK& k, hipStream_t s) {
 }
 
 template <int B, int C>
-inline bool launch_fast_c(const OutlierK& k, hipStream_t s) {
-    int G, F;
-    if (!fast_tile_for(k.n, &G, &F)) return false;
-    if (G == 1 && F == 16) launch_fast_shape<B, C, 1, 16>(k, s);
-    else if (G == 1 && F == 32) launch_fast_shape<B, C, 1, 32>(k, s);
-    else if (G == 2) launch_fast_shape<B, C, 2, 32>(k, s);
-    else if (G == 4 && F == 32) launch_fast_shape<B, C, 4, 32>(k, s);
-    else if (G == 8 && F == 32) launch_fast_shape<B, C, 8, 32>(k, s);
-    else launch_fast_shape<B, C, 8, 64>(k, s);
-    return true;
+inline bool launch_fast_c(const OutlierK& k, int shape, hipStream_t s) {
+    switch (shape) {
+    case 0: launch_fast_shape<B, C, 1, 16>(k, s); return true;
+    case 1: launch_fast_shape<B, C, 1, 32>(k, s); return true;
+    case 2: launch_fast_shape<B, C, 2, 16>(k, s); return true;
+    case 3: launch_fast_shape<B, C, 2, 32>(k, s); return true;
+    case 4: launch_fast_shape<B, C, 4, 16>(k, s); return true;
+    case 5: launch_fast_shape<B, C, 4, 32>(k, s); return true;
+    case 6: launch_fast_shape<B, C, 8, 16>(k, s); return true;
+    case 7: launch_fast_shape<B, C, 8, 32>(k, s); return true;
+    case 8: launch_fast_shape<B, C, 4, 64>(k, s); return true;
+    case 9: launch_fast_shape<B, C, 8, 64>(k, s); return true;
+    default: return false;
+    }
 }
 
-bool launch_outlier_fast_u8(const cp_stack* st, const OutlierK& k, hipStream_t s);
-bool launch_outlier_fast_u16(const cp_stack* st, const OutlierK& k, hipStream_t s);
+// One translation unit per (sample bytes, channels): kernels_fast_b<B>_c<C>.hip.
+bool launch_fast_b1_c3(const OutlierK& k, int shape, hipStream_t s);
+bool launch_fast_b1_c4(const OutlierK& k, int shape, hipStream_t s);
+bool launch_fast_b2_c3(const OutlierK& k, int shape, hipStream_t s);
+bool launch_fast_b2_c4(const OutlierK& k, int shape, hipStream_t s);
+
+inline bool launch_outlier_fast(uint32_t bps, uint32_t channels, const OutlierK& k, int shape, hipStream_t s) {
+    if (bps == 1 && channels == 3) return launch_fast_b1_c3(k, shape, s);
+    if (bps == 1 && channels == 4) return launch_fast_b1_c4(k, shape, s);
+    if (bps == 2 && channels == 3) return launch_fast_b2_c3(k, shape, s);
+    if (bps == 2 && channels == 4) return launch_fast_b2_c4(k, shape, s);
+    return false;
+}
 
 }  // namespace cp

@@ -52,6 +52,9 @@ struct cp_ctx {
     cp::ParamArena arena;
     cp::DeviceBuffer out_px, out_mask, out_stats, warn;  // host-output staging
     std::mutex mu;
+    // autotuned tile shape per (sample bytes - 1, channels - 1, frame bucket); -1 = default
+    int8_t tuned[2][4][8];
+    int shape_for(uint32_t bps, uint32_t channels, uint32_t n) const;
     int ensure(cp::DeviceBuffer& b, size_t bytes);
     int charge(uint64_t bytes);
     void uncharge(uint64_t bytes) { used = used >= bytes ? used - bytes : 0; }

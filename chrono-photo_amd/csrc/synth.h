@@ -27,6 +27,7 @@ typedef struct {
     int32_t blob_r, blob_val;
     int32_t shake_max;             // per-frame translation in [-shake_max, shake_max]
     int32_t texture_amp;           // static world-space 8x8 block texture amplitude
+    int32_t disk_y_lo, disk_y_hi;  // row band of the disk paths (0, 0: the whole frame)
 } cs_scene;
 
 CS_HD uint64_t cs_mix(uint64_t z) {
@@ -59,8 +60,10 @@ CS_HD void cs_disk(const cs_scene* s, uint32_t d, int32_t* r, int32_t* x0, int32
     uint64_t h = cs_hash(s->seed, 0xD15Cull, d, 0);
     *r = cs_uniform(h, s->disk_r_min, s->disk_r_max);
     int32_t W = (int32_t)s->width, H = (int32_t)s->height;
-    h = cs_mix(h); *y0 = cs_uniform(h, *r, H - 1 - *r);
-    h = cs_mix(h); *y1 = cs_uniform(h, *r, H - 1 - *r);
+    const int32_t ylo = s->disk_y_hi > s->disk_y_lo ? s->disk_y_lo : 0;
+    const int32_t yhi = s->disk_y_hi > s->disk_y_lo ? s->disk_y_hi : H - 1;
+    h = cs_mix(h); *y0 = cs_uniform(h, ylo + *r, yhi - *r);
+    h = cs_mix(h); *y1 = cs_uniform(h, ylo + *r, yhi - *r);
     *x0 = *r + W / 16;
     *x1 = W - 1 - *r - W / 16;
     if (d & 1u) { int32_t t = *x0; *x0 = *x1; *x1 = t; }

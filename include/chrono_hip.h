@@ -141,6 +141,8 @@ CP_API int cp_stack_push(cp_stack* st, const void* host_frames, uint32_t n, uint
 /* Declare slots [0, count) filled by a device-side writer (a decoder or generator writing
  * through cp_stack_info.device_ptr on the context stream). */
 CP_API int cp_stack_commit(cp_stack* st, uint32_t count);
+/* Copy frames [slot, slot+n) back to host memory (the inverse of cp_stack_upload). */
+CP_API int cp_stack_download(const cp_stack* st, uint32_t slot, void* host_frames, uint32_t n);
 CP_API int cp_stack_info_get(const cp_stack* st, cp_stack_info* out);
 CP_API int cp_stack_destroy(cp_stack* st);
 

@@ -234,18 +234,17 @@ def test_shake_crop_outlier_pipeline(ctx):
 # ------------------------------------------------------------------ synthetic scenes
 
 def test_device_generator_matches_host(ctx):
-    import torch
     for bps in (1, 2):
         scene = cpa.Scene(seed=9, width=200, height=150, bytes_per_sample=bps, n_frames=5,
                           base_lo=40 if bps == 1 else 8000, base_hi=200 if bps == 1 else 50000,
                           sigma=3.0 if bps == 1 else 200.0, n_disks=2, disk_r_min=8, disk_r_max=20,
                           disk_val_lo=15, disk_val_hi=25, blob_permille=300, shake_max=5, texture_amp=30)
         host = scene.generate()
-        dev = torch.zeros(host.nbytes, dtype=torch.uint8, device="cuda")
-        scene.generate_device(dev.data_ptr(), 0, 5)
-        torch.cuda.synchronize()
-        got = dev.cpu().numpy().view(host.dtype).reshape(host.shape)
-        assert np.array_equal(got, host), bps
+        st = cpa.Stack(ctx, 200, 150, 3, bps, capacity=5)
+        scene.generate_device(st.device_ptr, 0, 5, stream=ctx.stream)
+        ctx.synchronize()
+        st.commit(5)
+        assert np.array_equal(st.download(), host), bps
 
 
 def test_shake_bench_radii_mid_size(ctx):
