@@ -53,6 +53,11 @@ CP_ERR_CAPACITY = -8
 
 CP_FLAG_DEVICE_OUT = 1
 CP_FLAG_FORCE_GENERIC = 2
+CP_FLAG_NO_GRAPH = 4
+
+CP_MODE_OUTLIER = 0
+CP_MODE_LIGHTER = 1
+CP_MODE_DARKER = 2
 
 
 class cp_threshold(C.Structure):
@@ -126,7 +131,8 @@ ABI_SYMBOLS = [
     "cp_stack_commit", "cp_stack_download",
     "cp_stack_info_get", "cp_stack_destroy", "cp_threshold_new", "cp_threshold_blend_value",
     "cp_fade_table", "cp_fade_get", "cp_outlier", "cp_simple", "cp_shake", "cp_crop_create",
-    "cp_video_windows", "cp_batch_run", "cp_server_create", "cp_server_submit", "cp_server_poll",
+    "cp_video_windows", "cp_batch_run", "cp_batch_create", "cp_batch_launch", "cp_batch_destroy",
+    "cp_autotune", "cp_server_create", "cp_server_submit", "cp_server_poll",
     "cp_server_wait", "cp_server_destroy",
 ]
 
@@ -182,6 +188,10 @@ def load_library() -> C.CDLL:
         "cp_video_windows": (C.c_int, [P(cp_frame_range), P(cp_frame_range), C.c_uint32, P(C.c_int32),
                                        P(C.c_int32), P(C.c_int32), C.c_int32, P(C.c_int32)]),
         "cp_batch_run": (C.c_int, [C.c_void_p, P(cp_job), C.c_uint32, C.c_uint32]),
+        "cp_batch_create": (C.c_int, [C.c_void_p, P(cp_job), C.c_uint32, C.c_uint32, P(C.c_void_p)]),
+        "cp_batch_launch": (C.c_int, [C.c_void_p]),
+        "cp_batch_destroy": (C.c_int, [C.c_void_p]),
+        "cp_autotune": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, P(C.c_int32)]),
         "cp_server_create": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, P(C.c_void_p)]),
         "cp_server_submit": (C.c_int, [C.c_void_p, P(cp_job), P(C.c_uint64)]),
         "cp_server_poll": (C.c_int, [C.c_void_p, C.c_uint64, P(C.c_int32)]),
@@ -738,3 +748,82 @@ class Scene:
         out = np.zeros((n, 2), np.int32)
         load_synth().cs_shake_truth(C.byref(self.native()), n, out.ctypes.data_as(C.POINTER(C.c_int32)))
         return out
+
+
+# ------------------------------------------------------------------- batching
+
+class Job:
+    """One composite for a batch or the server: a mode, a stack, a frame selection and
+    device output pointers (out_px, out_mask, warnings: device addresses or None)."""
+
+    def __init__(self, mode: int, stack: Stack, out_px: int, out_mask: Optional[int] = None,
+                 warnings: Optional[int] = None, processor=None, image_indices=None, crop=None):
+        self.sel = _FrameSel(stack, image_indices, crop)
+        self.stack = stack
+        self.processor = processor
+        self.c = cp_job()
+        self.c.mode = mode
+        self.c.stack = stack.handle
+        self.c.frames = self.sel.c
+        if mode == CP_MODE_OUTLIER:
+            self.c.outlier = processor.params(stack.bps)
+        else:
+            proc = processor or SimpleProcessor(darker=mode == CP_MODE_DARKER)
+            self.processor = proc
+            for i in range(4):
+                self.c.weights[i] = float(proc.weights[i])
+            self.c.fade = proc.fade.native()
+        self.c.out = cp_outputs(out_px, out_mask, warnings, None)
+
+    def out_shape(self):
+        return self.sel.out_shape(self.stack)
+
+
+def _job_array(jobs):
+    arr = (cp_job * len(jobs))()
+    for i, j in enumerate(jobs):
+        arr[i] = j.c
+    return arr
+
+
+class Batch:
+    """cp_batch: many composites in one persistent launch, captured in a HIP graph."""
+
+    def __init__(self, ctx: Context, jobs, graph: bool = True):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.jobs = list(jobs)
+        self.handle = None
+        arr = _job_array(self.jobs)
+        h = C.c_void_p()
+        _check(self.lib.cp_batch_create(ctx.handle, arr, len(self.jobs), 0 if graph else CP_FLAG_NO_GRAPH,
+                                        C.byref(h)))
+        self.handle = h
+        ctx._children.add(self)
+
+    def launch(self):
+        _check(self.lib.cp_batch_launch(self.handle))
+
+    def close(self):
+        if self.handle and self.ctx.handle:
+            self.lib.cp_batch_destroy(self.handle)
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def batch_run(ctx: Context, jobs):
+    arr = _job_array(list(jobs))
+    _check(ctx.lib.cp_batch_run(ctx.handle, arr, len(arr), 0))
+
+
+def autotune(ctx: Context, bytes_per_sample: int, channels: int):
+    """Runs the tile-shape autotuner; returns [(lanes_per_pixel, frames_per_lane)] per bucket
+    (<=16, 32, 64, 128, 256, 512 frames)."""
+    out = (C.c_int32 * 6)()
+    _check(ctx.lib.cp_autotune(ctx.handle, bytes_per_sample, channels, out))
+    return [(v >> 8, v & 0xFF) for v in out]

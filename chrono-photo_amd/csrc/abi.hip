@@ -118,6 +118,61 @@ FadeK fade_to_device(cp_ctx* ctx, const cp_fade* f, int* rc) {
     return k;
 }
 
+int check_weights(const float w[4]) {
+    for (int i = 0; i < 4; ++i)
+        if (std::isnan(w[i]) || std::isinf(w[i])) return fail(CP_ERR_INVALID_ARG, "weights must be finite");
+    return CP_OK;
+}
+
+int prepare_outlier(const cp_stack* st, const cp_frames* frames, const cp_outlier_params* p, OutlierPrep* out) {
+    if (p->background < 0 || p->background > 3) return fail(CP_ERR_INVALID_ARG, "bad background mode");
+    if (p->outlier < 0 || p->outlier > 5) return fail(CP_ERR_INVALID_ARG, "bad outlier mode");
+    int rc = check_weights(p->weights);
+    if (rc) return rc;
+    Geometry g;
+    if ((rc = resolve_frames(st, frames, &g))) return rc;
+    const uint32_t S = p->sample_pos ? p->n_sample : g.n;
+    if (S == 0) return fail(CP_ERR_INVALID_ARG, "empty median sample");
+    if (!p->threshold.absolute && S < 3)
+        return fail(CP_ERR_TOO_FEW_FRAMES_REL, "relative thresholds need at least 3 samples (got %u)", S);
+    if (p->sample_pos) {
+        out->sample_mask.assign((g.n + 31) / 32, 0u);
+        uint32_t prev = 0;
+        for (uint32_t i = 0; i < p->n_sample; ++i) {
+            const uint32_t t = p->sample_pos[i];
+            if (t >= g.n || (i > 0 && t <= prev))
+                return fail(CP_ERR_INVALID_ARG, "sample positions must be sorted, unique and < n");
+            out->sample_mask[t >> 5] |= 1u << (t & 31);
+            prev = t;
+        }
+    }
+    if (p->fade.is_none == 0) {
+        if (p->fade.len == 0 || !p->fade.values) return fail(CP_ERR_INVALID_ARG, "fade table is empty");
+        out->fade_values.assign(p->fade.values, p->fade.values + p->fade.len);
+    }
+    OutlierK& k = out->k;
+    k = OutlierK{};
+    k.stack = st->dev;
+    k.n = g.n;
+    k.n_sample = S;
+    k.stack_w = st->width;
+    k.out_w = g.out_w;
+    k.out_h = g.out_h;
+    k.frame_offset = frames->frame_offset;
+    k.th = ThresholdK{p->threshold.absolute, p->threshold.min, p->threshold.max, p->threshold.scale};
+    k.background = p->background;
+    k.outlier = p->outlier;
+    for (int i = 0; i < 4; ++i) k.w[i] = p->weights[i];
+    k.fade.is_none = p->fade.is_none ? 1 : 0;
+    k.fade.repeat = p->fade.repeat;
+    k.fade.absolute = p->fade.absolute;
+    k.fade.offset = p->fade.offset;
+    k.fade.len = p->fade.len;
+    k.seed = p->seed;
+    out->pos_off = std::move(g.pos_off);
+    return CP_OK;
+}
+
 }  // namespace cp
 
 int cp_ctx::ensure(cp::DeviceBuffer& b, size_t bytes) {
@@ -384,63 +439,29 @@ CP_API float cp_fade_get(const cp_fade* f, int32_t frame) {
     return f->values[i];
 }
 
-static int check_weights(const float w[4]) {
-    for (int i = 0; i < 4; ++i)
-        if (std::isnan(w[i]) || std::isinf(w[i]))
-            return fail(CP_ERR_INVALID_ARG, "weights must be finite");
-    return CP_OK;
-}
-
 CP_API int cp_outlier(cp_ctx* ctx, const cp_stack* st, const cp_frames* frames,
                       const cp_outlier_params* p, const cp_outputs* out, uint32_t flags) {
     if (!ctx || !st || !frames || !p || !out || !out->px)
         return fail(CP_ERR_INVALID_ARG, "null argument");
     if (st->ctx != ctx) return fail(CP_ERR_INVALID_ARG, "stack belongs to another context");
-    if (p->background < 0 || p->background > 3) return fail(CP_ERR_INVALID_ARG, "bad background mode");
-    if (p->outlier < 0 || p->outlier > 5) return fail(CP_ERR_INVALID_ARG, "bad outlier mode");
-    int rc = check_weights(p->weights);
+    OutlierPrep prep;
+    int rc = prepare_outlier(st, frames, p, &prep);
     if (rc) return rc;
-    Geometry g;
-    if ((rc = resolve_frames(st, frames, &g))) return rc;
-    const uint32_t S = p->sample_pos ? p->n_sample : g.n;
-    if (S == 0) return fail(CP_ERR_INVALID_ARG, "empty median sample");
-    if (!p->threshold.absolute && S < 3)
-        return fail(CP_ERR_TOO_FEW_FRAMES_REL,
-                    "relative thresholds need at least 3 samples (got %u)", S);
     std::lock_guard<std::mutex> lock(ctx->mu);
     (void)hipSetDevice(ctx->device);
     hipStream_t s = ctx->stream;
-
-    OutlierK k{};
-    k.stack = st->dev;
-    k.pos_off = (const uint64_t*)ctx->arena.put(g.pos_off.data(), 8 * g.n, s);
+    OutlierK k = prep.k;
+    k.pos_off = (const uint64_t*)ctx->arena.put(prep.pos_off.data(), 8 * prep.pos_off.size(), s);
     if (!k.pos_off) return fail(CP_ERR_HIP, "parameter staging failed");
-    if (p->sample_pos) {
-        std::vector<uint32_t> mask((g.n + 31) / 32, 0u);
-        uint32_t prev = 0;
-        for (uint32_t i = 0; i < p->n_sample; ++i) {
-            uint32_t t = p->sample_pos[i];
-            if (t >= g.n || (i > 0 && t <= prev))
-                return fail(CP_ERR_INVALID_ARG, "sample positions must be sorted, unique and < n");
-            mask[t >> 5] |= 1u << (t & 31);
-            prev = t;
-        }
-        k.sample_mask = (const uint32_t*)ctx->arena.put(mask.data(), 4 * mask.size(), s);
+    if (!prep.sample_mask.empty()) {
+        k.sample_mask = (const uint32_t*)ctx->arena.put(prep.sample_mask.data(), 4 * prep.sample_mask.size(), s);
         if (!k.sample_mask) return fail(CP_ERR_HIP, "parameter staging failed");
     }
-    k.n = g.n;
-    k.n_sample = S;
-    k.stack_w = st->width;
-    k.out_w = g.out_w;
-    k.out_h = g.out_h;
-    k.frame_offset = frames->frame_offset;
-    k.th = ThresholdK{p->threshold.absolute, p->threshold.min, p->threshold.max, p->threshold.scale};
-    k.background = p->background;
-    k.outlier = p->outlier;
-    for (int i = 0; i < 4; ++i) k.w[i] = p->weights[i];
-    k.fade = fade_to_device(ctx, &p->fade, &rc);
-    if (rc) return rc;
-    k.seed = p->seed;
+    if (!prep.fade_values.empty()) {
+        k.fade.values = (const float*)ctx->arena.put(prep.fade_values.data(), 4 * prep.fade_values.size(), s);
+        if (!k.fade.values) return fail(CP_ERR_HIP, "parameter staging failed");
+    }
+    struct { uint32_t out_w, out_h; } g{k.out_w, k.out_h};
 
     const uint64_t npx = (uint64_t)g.out_w * g.out_h;
     const uint64_t px_bytes = npx * st->channels * st->bps;

@@ -309,16 +309,25 @@ __device__ __forceinline__ void outlier_tile(const OutlierK& k, uint32_t bx, uin
     // ---- medians / inverse IQR per weighted channel
     const RankPlan plan = make_rank_plan(N, ABS);
     float med[4] = {0, 0, 0, 0}, iqr_inv[4] = {0, 0, 0, 0};
-#pragma unroll
+    // One code path serves every channel (a rolled loop keeps the kernel small); the
+    // channel's registers are selected into `cur` with static register indices.
+#pragma unroll 1
     for (int c = 0; c < C; ++c) {
         if (k.w[c] == 0.0f) continue;
+        uint32_t cur[NW];
+#pragma unroll
+        for (int i = 0; i < NW; ++i) cur[i] = raw[0][i];
+#pragma unroll
+        for (int cc = 1; cc < C; ++cc)
+#pragma unroll
+            for (int i = 0; i < NW; ++i) cur[i] = c == cc ? raw[cc][i] : cur[i];
         uint32_t vals[6] = {0, 0, 0, 0, 0, 0};
         uint32_t m0, m1;
-        rank_pair<B, G, NW, ABS>(raw[c], plan.m0, plan.m1, ntot, m0, m1);
+        rank_pair<B, G, NW, ABS>(cur, plan.m0, plan.m1, ntot, m0, m1);
         uint32_t a0 = 0, a1 = 0, b0 = 0, b1 = 0;
         if constexpr (!ABS) {
-            rank_pair<B, G, NW, false>(raw[c], plan.a0, plan.a1, ntot, a0, a1);
-            rank_pair<B, G, NW, false>(raw[c], plan.b0, plan.b1, ntot, b0, b1);
+            rank_pair<B, G, NW, false>(cur, plan.a0, plan.a1, ntot, a0, a1);
+            rank_pair<B, G, NW, false>(cur, plan.b0, plan.b1, ntot, b0, b1);
         }
 #pragma unroll
         for (int i = 0; i < 6; ++i) {
@@ -332,7 +341,13 @@ __device__ __forceinline__ void outlier_tile(const OutlierK& k, uint32_t bx, uin
             else if (rk == plan.b1) v = b1;
             vals[i] = v;
         }
-        stats_from_ranks(plan, vals, ABS, &med[c], &iqr_inv[c]);
+        float mc, ic;
+        stats_from_ranks(plan, vals, ABS, &mc, &ic);
+#pragma unroll
+        for (int cc = 0; cc < C; ++cc) {
+            med[cc] = c == cc ? mc : med[cc];
+            iqr_inv[cc] = c == cc ? ic : iqr_inv[cc];
+        }
     }
 
     // ---- classification over the lane's frames

@@ -80,6 +80,17 @@ struct Geometry {
 };
 
 int resolve_frames(const cp_stack* st, const cp_frames* fr, Geometry* g);
+int check_weights(const float w[4]);
+
+// Validated arguments of one outlier composite; device pointers (pos_off, sample_mask,
+// fade.values, outputs) are left for the caller to place.
+struct OutlierPrep {
+    OutlierK k;
+    std::vector<uint64_t> pos_off;
+    std::vector<uint32_t> sample_mask;  // empty: every frame is in the median sample
+    std::vector<float> fade_values;     // empty: no fade
+};
+int prepare_outlier(const cp_stack* st, const cp_frames* frames, const cp_outlier_params* p, OutlierPrep* out);
 FadeK fade_to_device(cp_ctx* ctx, const cp_fade* f, int* rc);
 
 // Kernel launchers (defined in the kernel translation units).

@@ -200,7 +200,27 @@ typedef struct {
     cp_outputs out;             /* device pointers */
 } cp_job;
 
+typedef struct cp_batch cp_batch;
+
+/* Validate a list of jobs, stage all their per-call tables in device memory owned by the
+ * batch, and capture the launch sequence (one persistent kernel per sample width and
+ * channel count) in a HIP graph.  Outlier jobs with up to 256 frames, 3-4 channels and no
+ * sample subset run in the persistent kernel; every other job runs as its own launch.  The
+ * caller's arrays are copied; output pointers must stay valid while the batch exists. */
+CP_API int cp_batch_create(cp_ctx* ctx, const cp_job* jobs, uint32_t n_jobs, uint32_t flags,
+                           cp_batch** out);
+/* Replay the batch on the context stream (asynchronous). */
+CP_API int cp_batch_launch(cp_batch* batch);
+CP_API int cp_batch_destroy(cp_batch* batch);
+/* create + launch + wait + destroy. */
 CP_API int cp_batch_run(cp_ctx* ctx, const cp_job* jobs, uint32_t n_jobs, uint32_t flags);
+#define CP_FLAG_NO_GRAPH 4u /* cp_batch_create: launch kernels directly, no graph capture */
+
+/* Autotuner: for each frame-count bucket of the register-resident outlier kernel
+ * (<=16, 32, 64, 128, 256, 512 frames) time the candidate tile shapes on a synthetic stack
+ * of this sample width and channel count and keep the fastest for the context.  shapes
+ * (nullable, 6 entries) receives (lanes_per_pixel << 8) | frames_per_lane per bucket. */
+CP_API int cp_autotune(cp_ctx* ctx, uint32_t bytes_per_sample, uint32_t channels, int32_t* shapes);
 
 /* -------------------------------------------------------------------- serving
  * Continuous-batching request loop over resident stacks (no reference counterpart: the

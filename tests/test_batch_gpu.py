@@ -1,0 +1,118 @@
+"""Batched launches (one persistent kernel over many stacks, captured in a HIP graph) give
+byte-identical results to one launch per stack, and the autotuner's tile choice never
+changes results."""
+import numpy as np
+import pytest
+
+import chrono_photo_amd as cpa
+from chrono_photo_amd import hip as H
+import oracle_py as O
+
+pytestmark = pytest.mark.gpu
+
+SELS = list(cpa.OutlierSelectionMode)
+
+
+def scene_stack(ctx, seed, n, w, h, c=3, bps=1):
+    sc = cpa.Scene(seed=seed, width=w, height=h, channels=c, bytes_per_sample=bps, n_frames=n,
+                   base_lo=40 if bps == 1 else 8000, base_hi=200 if bps == 1 else 50000,
+                   sigma=3.0 if bps == 1 else 200.0, n_disks=2, disk_r_min=4, disk_r_max=9,
+                   disk_val_lo=15 if bps == 1 else 2000, disk_val_hi=25 if bps == 1 else 3000,
+                   blob_permille=100, blob_r=6, blob_val=230 if bps == 1 else 60000)
+    x = sc.generate()
+    return x, cpa.Stack.from_array(ctx, x)
+
+
+def make_jobs(ctx, specs):
+    jobs, bufs = [], []
+    for (seed, n, w, h, c, bps, mode, absolute, sel, bg) in specs:
+        x, st = scene_stack(ctx, seed, n, w, h, c, bps)
+        px = H.DeviceBuffer(w * h * c * bps)
+        mask = H.DeviceBuffer(w * h * c)
+        warn = H.DeviceBuffer(8)
+        warn.from_host(np.zeros(1, np.uint64))
+        if mode == cpa.CP_MODE_OUTLIER:
+            thr = cpa.Threshold.abs(0.05, 0.2) if absolute else cpa.Threshold.rel(2.0, 4.0)
+            proc = cpa.OutlierProcessor(thr, bg, sel)
+            job = cpa.Job(mode, st, px.ptr, mask.ptr, warn.ptr, processor=proc)
+        else:
+            job = cpa.Job(mode, st, px.ptr)
+        jobs.append(job)
+        bufs.append((x, st, px, mask, warn))
+    return jobs, bufs
+
+
+def read(bufs, job):
+    x, st, px, mask, warn = bufs
+    h, w, c = job.out_shape()
+    img = px.to_host(np.zeros((h, w, c), x.dtype))
+    m = mask.to_host(np.zeros((h, w, c), np.uint8))
+    return img, m
+
+
+def specs_mixed():
+    rng = np.random.default_rng(3)
+    specs = []
+    for i in range(14):
+        n = int(rng.integers(8, 200))
+        w = int(rng.integers(40, 140))
+        h = int(rng.integers(20, 60))
+        specs.append((100 + i, n, w, h, 3 if i % 5 else 4, 1, cpa.CP_MODE_OUTLIER, i % 2 == 0,
+                      SELS[i % 6], i % 4))
+    specs.append((200, 40, 64, 32, 3, 2, cpa.CP_MODE_OUTLIER, True, SELS[4], 0))   # u16: own launch
+    specs.append((201, 300, 48, 20, 3, 1, cpa.CP_MODE_OUTLIER, True, SELS[2], 0))  # > 256 frames
+    specs.append((202, 30, 50, 30, 3, 1, cpa.CP_MODE_DARKER, True, SELS[0], 0))
+    specs.append((203, 30, 50, 30, 4, 1, cpa.CP_MODE_LIGHTER, True, SELS[0], 0))
+    return specs
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_batch_matches_plain_launches(ctx, graph):
+    specs = specs_mixed()
+    jobs, bufs = make_jobs(ctx, specs)
+    batch = cpa.Batch(ctx, jobs, graph=graph)
+    for rep in range(3):  # graph replays, with eager work between them
+        for b in bufs:
+            b[2].from_host(np.zeros(b[2].nbytes, np.uint8))
+        batch.launch()
+        ctx.synchronize()
+        for spec, job, b in zip(specs, jobs, bufs):
+            img, m = read(b, job)
+            x, st = b[0], b[1]
+            if spec[6] == cpa.CP_MODE_OUTLIER:
+                ref_img, ref_mask = job.processor.process(st)
+                assert np.array_equal(img, ref_img), (rep, spec)
+                assert np.array_equal(m, ref_mask), (rep, spec)
+            else:
+                ref = cpa.SimpleProcessor(darker=spec[6] == cpa.CP_MODE_DARKER).process(st)
+                assert np.array_equal(img, ref), (rep, spec)
+    batch.close()
+
+
+def test_batch_against_oracle(ctx):
+    specs = specs_mixed()[:6]
+    jobs, bufs = make_jobs(ctx, specs)
+    cpa.batch_run(ctx, jobs)
+    for spec, job, b in zip(specs, jobs, bufs):
+        img, m = read(b, job)
+        x = b[0]
+        _, n, w, h, c, bps, mode, absolute, sel, bg = spec
+        thr = O.threshold(absolute, 0.05 if absolute else 2.0, 0.2 if absolute else 4.0, bps)
+        oimg, omask, _ = O.outlier(x, thr, bg, int(sel))
+        assert np.array_equal(img, oimg), spec
+        assert np.array_equal(m, omask), spec
+
+
+@pytest.mark.parametrize("bps", [1, 2])
+def test_autotune_keeps_results(ctx, bps):
+    shapes = cpa.autotune(ctx, bps, 3)
+    assert len(shapes) == 6
+    for (g, f), nmax in zip(shapes, (16, 32, 64, 128, 256, 512)):
+        assert g * f >= nmax and g in (1, 2, 4, 8)
+    for n in (12, 30, 60, 120, 250):
+        x, st = scene_stack(ctx, 7 + n, n, 70, 24, 3, bps)
+        proc = cpa.OutlierProcessor(cpa.Threshold.abs(0.05, 0.2), 0, 4)
+        img, mask = proc.process(st)
+        oimg, omask, _ = O.outlier(x, O.threshold(True, 0.05, 0.2, bps), 0, 4)
+        assert np.array_equal(img, oimg), n
+        assert np.array_equal(mask, omask), n
