@@ -114,6 +114,17 @@ class cp_job(C.Structure):
                 ("out", cp_outputs)]
 
 
+class cp_request(C.Structure):
+    _fields_ = [("job", cp_job), ("host_frames", C.c_void_p), ("width", C.c_uint32), ("height", C.c_uint32),
+                ("channels", C.c_uint32), ("bytes_per_sample", C.c_uint32), ("n_frames", C.c_uint32),
+                ("host_out_px", C.c_void_p), ("host_out_mask", C.c_void_p)]
+
+
+class cp_server_stats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("rejected", C.c_uint64),
+                ("pixel_samples", C.c_uint64), ("peak_bytes", C.c_uint64), ("peak_inflight", C.c_uint32)]
+
+
 class cs_scene(C.Structure):
     _fields_ = [("seed", C.c_uint64), ("width", C.c_uint32), ("height", C.c_uint32),
                 ("channels", C.c_uint32), ("bytes_per_sample", C.c_uint32), ("n_frames", C.c_uint32),
@@ -133,7 +144,7 @@ ABI_SYMBOLS = [
     "cp_fade_table", "cp_fade_get", "cp_outlier", "cp_simple", "cp_shake", "cp_crop_create",
     "cp_video_windows", "cp_batch_run", "cp_batch_create", "cp_batch_launch", "cp_batch_destroy",
     "cp_autotune", "cp_server_create", "cp_server_submit", "cp_server_poll",
-    "cp_server_wait", "cp_server_destroy",
+    "cp_server_wait", "cp_server_stats_get", "cp_server_destroy",
 ]
 
 _lib = None
@@ -193,7 +204,8 @@ def load_library() -> C.CDLL:
         "cp_batch_destroy": (C.c_int, [C.c_void_p]),
         "cp_autotune": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, P(C.c_int32)]),
         "cp_server_create": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, P(C.c_void_p)]),
-        "cp_server_submit": (C.c_int, [C.c_void_p, P(cp_job), P(C.c_uint64)]),
+        "cp_server_submit": (C.c_int, [C.c_void_p, P(cp_request), P(C.c_uint64)]),
+        "cp_server_stats_get": (C.c_int, [C.c_void_p, P(cp_server_stats)]),
         "cp_server_poll": (C.c_int, [C.c_void_p, C.c_uint64, P(C.c_int32)]),
         "cp_server_wait": (C.c_int, [C.c_void_p, C.c_uint64, P(C.c_double)]),
         "cp_server_destroy": (C.c_int, [C.c_void_p]),
@@ -557,7 +569,7 @@ class Stack:
 class _FrameSel:
     """Keeps the arrays behind a cp_frames alive."""
 
-    def __init__(self, stack: Stack, image_indices=None, crop=None, n=None):
+    def __init__(self, stack: Optional[Stack], image_indices=None, crop=None, n=None):
         if image_indices is None:
             n = stack.count if n is None else n
             self.idx = None
@@ -756,17 +768,19 @@ class Job:
     """One composite for a batch or the server: a mode, a stack, a frame selection and
     device output pointers (out_px, out_mask, warnings: device addresses or None)."""
 
-    def __init__(self, mode: int, stack: Stack, out_px: int, out_mask: Optional[int] = None,
-                 warnings: Optional[int] = None, processor=None, image_indices=None, crop=None):
-        self.sel = _FrameSel(stack, image_indices, crop)
+    def __init__(self, mode: int, stack: Optional[Stack], out_px: Optional[int], out_mask: Optional[int] = None,
+                 warnings: Optional[int] = None, processor=None, image_indices=None, crop=None,
+                 n_frames: Optional[int] = None, bytes_per_sample: Optional[int] = None):
+        self.sel = _FrameSel(stack, image_indices, crop, n_frames)
         self.stack = stack
         self.processor = processor
         self.c = cp_job()
         self.c.mode = mode
-        self.c.stack = stack.handle
+        self.c.stack = stack.handle if stack is not None else None
         self.c.frames = self.sel.c
+        bps = stack.bps if stack is not None else bytes_per_sample
         if mode == CP_MODE_OUTLIER:
-            self.c.outlier = processor.params(stack.bps)
+            self.c.outlier = processor.params(bps)
         else:
             proc = processor or SimpleProcessor(darker=mode == CP_MODE_DARKER)
             self.processor = proc
@@ -827,3 +841,67 @@ def autotune(ctx: Context, bytes_per_sample: int, channels: int):
     out = (C.c_int32 * 6)()
     _check(ctx.lib.cp_autotune(ctx.handle, bytes_per_sample, channels, out))
     return [(v >> 8, v & 0xFF) for v in out]
+
+
+# -------------------------------------------------------------------- serving
+
+class Server:
+    """cp_server: continuous-batching request loop (HBM-budget admission, stream slots)."""
+
+    def __init__(self, ctx: Context, hbm_budget: int = 0, max_inflight: int = 4):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.handle = None
+        self._keep = {}
+        h = C.c_void_p()
+        _check(self.lib.cp_server_create(ctx.handle, hbm_budget, max_inflight, C.byref(h)))
+        self.handle = h
+        ctx._children.add(self)
+
+    def submit(self, job: Job, host_frames=None, host_out=None, host_mask=None) -> int:
+        """Queues `job`; with host_frames ([n][h][w][c] array) the server uploads them into
+        its own device memory.  host_out / host_mask receive the image / blend mask."""
+        r = cp_request()
+        r.job = job.c
+        if host_frames is not None:
+            host_frames = np.ascontiguousarray(host_frames)
+            n, hh, ww, cc = host_frames.shape
+            r.host_frames = host_frames.ctypes.data
+            r.width, r.height, r.channels = ww, hh, cc
+            r.bytes_per_sample = host_frames.dtype.itemsize
+            r.n_frames = n
+        r.host_out_px = None if host_out is None else host_out.ctypes.data
+        r.host_out_mask = None if host_mask is None else host_mask.ctypes.data
+        t = C.c_uint64()
+        _check(self.lib.cp_server_submit(self.handle, C.byref(r), C.byref(t)))
+        self._keep[t.value] = (job, host_frames, host_out, host_mask, r)
+        return t.value
+
+    def poll(self, ticket: int) -> bool:
+        d = C.c_int32()
+        _check(self.lib.cp_server_poll(self.handle, ticket, C.byref(d)))
+        return bool(d.value)
+
+    def wait(self, ticket: int) -> float:
+        """Blocks until the request is done; returns its latency in ms."""
+        ms = C.c_double()
+        rc = self.lib.cp_server_wait(self.handle, ticket, C.byref(ms))
+        self._keep.pop(ticket, None)
+        _check(rc)
+        return ms.value
+
+    def stats(self) -> dict:
+        st = cp_server_stats()
+        _check(self.lib.cp_server_stats_get(self.handle, C.byref(st)))
+        return {f: getattr(st, f) for f, _ in cp_server_stats._fields_}
+
+    def close(self):
+        if self.handle and self.ctx.handle:
+            self.lib.cp_server_destroy(self.handle)
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,364 @@
+// server.hip -- continuous-batching request loop (cp_server_*).
+//
+// A worker thread owns max_inflight stream slots.  Requests queue in arrival order; the
+// worker admits the queue head whenever a slot is free and the server's HBM budget covers
+// the request's device memory (its uploaded frames and server-side outputs), starts it on
+// the slot's stream (stream-ordered allocation, upload, composite, download, completion
+// event) and retires finished requests by polling their events.  Admitted requests run
+// concurrently on different streams; each request's arithmetic is exactly that of a plain
+// call, so results do not depend on what else is in flight.
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <memory>
+#include <thread>
+#include <unordered_map>
+
+#include "runtime.h"
+
+using namespace cp;
+using Clock = std::chrono::steady_clock;
+
+namespace {
+
+struct Request {
+    uint64_t ticket = 0;
+    cp_request req{};
+    std::vector<uint32_t> frames, sample;
+    std::vector<int32_t> crop;
+    std::vector<float> fade_o, fade_s;
+    uint64_t need = 0;
+    Clock::time_point t_submit, t_done;
+    int status = 1;  // 1: pending, 0: done, < 0: error
+    std::string error;
+    cp_stack own_stack;
+    void* own_frames = nullptr;
+    void* own_px = nullptr;
+    void* own_mask = nullptr;
+    uint64_t pixel_samples = 0;
+};
+
+struct Slot {
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    ParamArena arena;
+    Request* cur = nullptr;
+};
+
+uint64_t request_bytes(const cp_request& r, uint32_t* out_w, uint32_t* out_h) {
+    const cp_stack* st = r.job.stack;
+    const uint32_t W = st ? st->width : r.width, H = st ? st->height : r.height;
+    const uint32_t C = st ? st->channels : r.channels, B = st ? st->bps : r.bytes_per_sample;
+    *out_w = r.job.frames.out_width ? r.job.frames.out_width : W;
+    *out_h = r.job.frames.out_height ? r.job.frames.out_height : H;
+    uint64_t bytes = 0;
+    if (!st) bytes += (uint64_t)W * H * C * B * r.n_frames;
+    if (!r.job.out.px) bytes += (uint64_t)(*out_w) * (*out_h) * C * B;
+    if (!r.job.out.mask && r.host_out_mask) bytes += (uint64_t)(*out_w) * (*out_h) * C;
+    return bytes;
+}
+
+}  // namespace
+
+struct cp_server {
+    cp_ctx* ctx = nullptr;
+    uint64_t budget = 0;
+    std::vector<Slot> slots;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<Request*> queue;
+    std::unordered_map<uint64_t, std::unique_ptr<Request>> all;
+    uint64_t used = 0, next = 1;
+    bool stop = false;
+    std::thread worker;
+    cp_server_stats stats{};
+
+    int start(Request* r, Slot& slot);
+    void retire(Slot& slot, int status);
+    void loop();
+};
+
+int cp_server::start(Request* r, Slot& slot) {
+    hipStream_t s = slot.stream;
+    cp_request& q = r->req;
+    cp_job& j = q.job;
+    const cp_stack* st = j.stack;
+    if (!st) {
+        const uint64_t bytes = (uint64_t)q.width * q.height * q.channels * q.bytes_per_sample * q.n_frames;
+        if (hipMallocAsync(&r->own_frames, bytes, s) != hipSuccess)
+            return fail(CP_ERR_HIP, "server: frame allocation of %llu bytes failed", (unsigned long long)bytes);
+        if (hipMemcpyAsync(r->own_frames, q.host_frames, bytes, hipMemcpyHostToDevice, s) != hipSuccess)
+            return fail(CP_ERR_HIP, "server: frame upload failed");
+        cp_stack& o = r->own_stack;
+        o.ctx = ctx;
+        o.width = q.width;
+        o.height = q.height;
+        o.channels = q.channels;
+        o.bps = q.bytes_per_sample;
+        o.capacity = o.count = q.n_frames;
+        o.dev = (uint8_t*)r->own_frames;
+        o.owned = false;
+        st = &o;
+    }
+    uint32_t ow = 0, oh = 0;
+    (void)request_bytes(q, &ow, &oh);
+    const uint64_t px_bytes = (uint64_t)ow * oh * st->channels * st->bps;
+    const uint64_t mask_bytes = (uint64_t)ow * oh * st->channels;
+    void* px = j.out.px;
+    uint8_t* mask = j.out.mask;
+    if (!px) {
+        if (hipMallocAsync(&r->own_px, px_bytes, s) != hipSuccess) return fail(CP_ERR_HIP, "server: output allocation failed");
+        px = r->own_px;
+    }
+    if (!mask && q.host_out_mask) {
+        if (hipMallocAsync(&r->own_mask, mask_bytes, s) != hipSuccess) return fail(CP_ERR_HIP, "server: mask allocation failed");
+        mask = (uint8_t*)r->own_mask;
+    }
+    int rc;
+    if (j.mode == CP_MODE_OUTLIER) {
+        OutlierPrep prep;
+        if ((rc = prepare_outlier(st, &j.frames, &j.outlier, &prep))) return rc;
+        OutlierK k = prep.k;
+        k.pos_off = (const uint64_t*)slot.arena.put(prep.pos_off.data(), 8 * prep.pos_off.size(), s);
+        if (!prep.sample_mask.empty())
+            k.sample_mask = (const uint32_t*)slot.arena.put(prep.sample_mask.data(), 4 * prep.sample_mask.size(), s);
+        if (!prep.fade_values.empty())
+            k.fade.values = (const float*)slot.arena.put(prep.fade_values.data(), 4 * prep.fade_values.size(), s);
+        if (!k.pos_off) return fail(CP_ERR_HIP, "server: parameter staging failed");
+        k.out_px = px;
+        k.out_mask = mask;
+        k.warnings = (unsigned long long*)j.out.warnings;
+        k.stats = (PixelStats*)j.out.stats;
+        if ((rc = launch_outlier(ctx, st, k, 0, s))) return rc;
+        r->pixel_samples = (uint64_t)k.n * k.out_w * k.out_h;
+    } else {
+        if ((rc = check_weights(j.weights))) return rc;
+        Geometry g;
+        if ((rc = resolve_frames(st, &j.frames, &g))) return rc;
+        SimpleK k{};
+        k.stack = st->dev;
+        k.pos_off = (const uint64_t*)slot.arena.put(g.pos_off.data(), 8 * g.n, s);
+        if (!k.pos_off) return fail(CP_ERR_HIP, "server: parameter staging failed");
+        k.n = g.n;
+        k.stack_w = st->width;
+        k.out_w = g.out_w;
+        k.out_h = g.out_h;
+        k.frame_offset = j.frames.frame_offset;
+        k.darker = j.mode == CP_MODE_DARKER;
+        for (int i = 0; i < 4; ++i) k.w[i] = j.weights[i];
+        k.fade.is_none = j.fade.is_none ? 1 : 0;
+        if (!j.fade.is_none) {
+            k.fade.repeat = j.fade.repeat;
+            k.fade.absolute = j.fade.absolute;
+            k.fade.offset = j.fade.offset;
+            k.fade.len = j.fade.len;
+            k.fade.values = (const float*)slot.arena.put(j.fade.values, 4 * j.fade.len, s);
+        }
+        k.out_px = px;
+        if ((rc = launch_simple(ctx, st, k, s))) return rc;
+        r->pixel_samples = (uint64_t)k.n * k.out_w * k.out_h;
+    }
+    if (q.host_out_px && hipMemcpyAsync(q.host_out_px, px, px_bytes, hipMemcpyDeviceToHost, s) != hipSuccess)
+        return fail(CP_ERR_HIP, "server: result download failed");
+    if (q.host_out_mask && mask &&
+        hipMemcpyAsync(q.host_out_mask, mask, mask_bytes, hipMemcpyDeviceToHost, s) != hipSuccess)
+        return fail(CP_ERR_HIP, "server: mask download failed");
+    if (hipEventRecord(slot.done, s) != hipSuccess) return fail(CP_ERR_HIP, "server: event record failed");
+    return CP_OK;
+}
+
+void cp_server::retire(Slot& slot, int status) {
+    Request* r = slot.cur;
+    hipStream_t s = slot.stream;
+    if (r->own_frames) (void)hipFreeAsync(r->own_frames, s);
+    if (r->own_px) (void)hipFreeAsync(r->own_px, s);
+    if (r->own_mask) (void)hipFreeAsync(r->own_mask, s);
+    r->own_frames = r->own_px = r->own_mask = nullptr;
+    used -= r->need;
+    r->t_done = Clock::now();
+    r->status = status;
+    if (status == CP_OK) {
+        stats.completed++;
+        stats.pixel_samples += r->pixel_samples;
+    }
+    slot.cur = nullptr;
+}
+
+void cp_server::loop() {
+    (void)hipSetDevice(ctx->device);
+    std::unique_lock<std::mutex> lock(mu);
+    for (;;) {
+        bool busy = false, retired = false;
+        for (auto& sl : slots) {
+            if (!sl.cur) continue;
+            hipError_t e = hipEventQuery(sl.done);
+            if (e == hipSuccess) {
+                retire(sl, CP_OK);
+                retired = true;
+            } else if (e != hipErrorNotReady) {
+                sl.cur->error = hipGetErrorString(e);
+                retire(sl, CP_ERR_HIP);
+                retired = true;
+            } else {
+                busy = true;
+            }
+        }
+        while (!queue.empty()) {
+            Slot* free_slot = nullptr;
+            for (auto& sl : slots)
+                if (!sl.cur) { free_slot = &sl; break; }
+            if (!free_slot) break;
+            Request* r = queue.front();
+            if (budget && r->need > budget) {
+                queue.pop_front();
+                r->status = CP_ERR_OUT_OF_HBM_BUDGET;
+                r->error = "request needs more device memory than the server budget";
+                r->t_done = Clock::now();
+                stats.rejected++;
+                retired = true;
+                continue;
+            }
+            if (budget && used + r->need > budget) break;  // wait for memory to free up
+            queue.pop_front();
+            used += r->need;
+            if (used > stats.peak_bytes) stats.peak_bytes = used;
+            free_slot->cur = r;
+            int rc = start(r, *free_slot);
+            if (rc != CP_OK) {
+                r->error = cp_last_error();
+                (void)hipStreamSynchronize(free_slot->stream);
+                retire(*free_slot, rc);
+                retired = true;
+                continue;
+            }
+            busy = true;
+            uint32_t inflight = 0;
+            for (auto& sl : slots) inflight += sl.cur != nullptr;
+            if (inflight > stats.peak_inflight) stats.peak_inflight = inflight;
+        }
+        if (retired) cv_done.notify_all();
+        if (stop && queue.empty() && !busy) break;
+        if (busy) cv_work.wait_for(lock, std::chrono::microseconds(20));
+        else cv_work.wait(lock, [&] { return stop || !queue.empty(); });
+    }
+}
+
+extern "C" CP_API int cp_server_create(cp_ctx* ctx, uint64_t hbm_budget_bytes, uint32_t max_inflight,
+                                       cp_server** out) {
+    if (!ctx || !out) return fail(CP_ERR_INVALID_ARG, "null argument");
+    if (max_inflight == 0) max_inflight = 4;
+    if (max_inflight > 16) return fail(CP_ERR_INVALID_ARG, "at most 16 in-flight slots");
+    auto srv = std::make_unique<cp_server>();
+    srv->ctx = ctx;
+    srv->budget = hbm_budget_bytes;
+    srv->slots.resize(max_inflight);
+    (void)hipSetDevice(ctx->device);
+    for (auto& sl : srv->slots) {
+        if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess || sl.arena.init(1u << 20) != 0) {
+            for (auto& t : srv->slots) {
+                if (t.stream) (void)hipStreamDestroy(t.stream);
+                if (t.done) (void)hipEventDestroy(t.done);
+                t.arena.release();
+            }
+            return fail(CP_ERR_HIP, "server: stream slot creation failed");
+        }
+    }
+    cp_server* s = srv.release();
+    s->worker = std::thread([s] { s->loop(); });
+    *out = s;
+    return CP_OK;
+}
+
+extern "C" CP_API int cp_server_submit(cp_server* srv, const cp_request* req, uint64_t* ticket) {
+    if (!srv || !req || !ticket) return fail(CP_ERR_INVALID_ARG, "null argument");
+    const cp_job& j = req->job;
+    if (j.mode < CP_MODE_OUTLIER || j.mode > CP_MODE_DARKER) return fail(CP_ERR_INVALID_ARG, "bad mode");
+    if (!j.stack) {
+        if (!req->host_frames || req->n_frames == 0 || req->width == 0 || req->height == 0)
+            return fail(CP_ERR_INVALID_ARG, "request without stack needs host frames");
+        if (req->channels < 1 || req->channels > 4 || (req->bytes_per_sample != 1 && req->bytes_per_sample != 2))
+            return fail(CP_ERR_INVALID_ARG, "bad frame layout");
+    } else if (j.stack->ctx != srv->ctx) {
+        return fail(CP_ERR_INVALID_ARG, "stack belongs to another context");
+    }
+    if (!j.out.px && !req->host_out_px) return fail(CP_ERR_INVALID_ARG, "request has no output");
+    auto r = std::make_unique<Request>();
+    r->req = *req;
+    const uint32_t n = j.frames.n;
+    cp_job& jj = r->req.job;
+    if (j.frames.frames) { r->frames.assign(j.frames.frames, j.frames.frames + n); jj.frames.frames = r->frames.data(); }
+    if (j.frames.crop_xy) { r->crop.assign(j.frames.crop_xy, j.frames.crop_xy + 2 * n); jj.frames.crop_xy = r->crop.data(); }
+    if (j.outlier.sample_pos) {
+        r->sample.assign(j.outlier.sample_pos, j.outlier.sample_pos + j.outlier.n_sample);
+        jj.outlier.sample_pos = r->sample.data();
+    }
+    if (!j.outlier.fade.is_none && j.outlier.fade.values) {
+        r->fade_o.assign(j.outlier.fade.values, j.outlier.fade.values + j.outlier.fade.len);
+        jj.outlier.fade.values = r->fade_o.data();
+    }
+    if (!j.fade.is_none && j.fade.values) {
+        r->fade_s.assign(j.fade.values, j.fade.values + j.fade.len);
+        jj.fade.values = r->fade_s.data();
+    }
+    uint32_t ow, oh;
+    r->need = request_bytes(r->req, &ow, &oh);
+    r->t_submit = Clock::now();
+    std::lock_guard<std::mutex> lock(srv->mu);
+    r->ticket = srv->next++;
+    *ticket = r->ticket;
+    srv->queue.push_back(r.get());
+    srv->all[r->ticket] = std::move(r);
+    srv->stats.submitted++;
+    srv->cv_work.notify_one();
+    return CP_OK;
+}
+
+extern "C" CP_API int cp_server_poll(cp_server* srv, uint64_t ticket, int32_t* done) {
+    if (!srv || !done) return fail(CP_ERR_INVALID_ARG, "null argument");
+    std::lock_guard<std::mutex> lock(srv->mu);
+    auto it = srv->all.find(ticket);
+    if (it == srv->all.end()) return fail(CP_ERR_INVALID_ARG, "unknown ticket");
+    *done = it->second->status != 1;
+    return CP_OK;
+}
+
+extern "C" CP_API int cp_server_wait(cp_server* srv, uint64_t ticket, double* latency_ms) {
+    if (!srv) return fail(CP_ERR_INVALID_ARG, "null argument");
+    std::unique_lock<std::mutex> lock(srv->mu);
+    auto it = srv->all.find(ticket);
+    if (it == srv->all.end()) return fail(CP_ERR_INVALID_ARG, "unknown ticket");
+    Request* r = it->second.get();
+    srv->cv_done.wait(lock, [&] { return r->status != 1; });
+    if (latency_ms) *latency_ms = std::chrono::duration<double, std::milli>(r->t_done - r->t_submit).count();
+    const int status = r->status;
+    std::string err = r->error;
+    srv->all.erase(it);
+    if (status != CP_OK) return fail(status, "request %llu: %s", (unsigned long long)ticket, err.c_str());
+    return CP_OK;
+}
+
+extern "C" CP_API int cp_server_stats_get(cp_server* srv, cp_server_stats* out) {
+    if (!srv || !out) return fail(CP_ERR_INVALID_ARG, "null argument");
+    std::lock_guard<std::mutex> lock(srv->mu);
+    *out = srv->stats;
+    return CP_OK;
+}
+
+extern "C" CP_API int cp_server_destroy(cp_server* srv) {
+    if (!srv) return CP_OK;
+    {
+        std::lock_guard<std::mutex> lock(srv->mu);
+        srv->stop = true;
+        srv->cv_work.notify_all();
+    }
+    if (srv->worker.joinable()) srv->worker.join();
+    for (auto& sl : srv->slots) {
+        (void)hipStreamSynchronize(sl.stream);
+        (void)hipStreamDestroy(sl.stream);
+        (void)hipEventDestroy(sl.done);
+        sl.arena.release();
+    }
+    delete srv;
+    return CP_OK;
+}

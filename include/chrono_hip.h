@@ -223,13 +223,38 @@ CP_API int cp_batch_run(cp_ctx* ctx, const cp_job* jobs, uint32_t n_jobs, uint32
 CP_API int cp_autotune(cp_ctx* ctx, uint32_t bytes_per_sample, uint32_t channels, int32_t* shapes);
 
 /* -------------------------------------------------------------------- serving
- * Continuous-batching request loop over resident stacks (no reference counterpart: the
- * reference's video mode recomputes every window, src/main.rs:254-330). */
+ * Continuous-batching request loop (no reference counterpart: the reference's video mode
+ * re-reads and recomputes every window from disk, src/main.rs:254-330).  A worker thread
+ * admits queued requests in arrival order while the server's HBM budget and its
+ * max_inflight stream slots allow, runs each on its own stream (upload, composite,
+ * download), so admitted requests overlap on the device, and retires them as their
+ * completion events fire.  Requests on resident stacks (e.g. the windows of a chrono-video
+ * over one uploaded frame sequence) need no upload. */
+typedef struct {
+    cp_job job;               /* job.stack NULL: the request brings host frames below */
+    const void* host_frames;  /* [n_frames][height][width][channels] */
+    uint32_t width, height, channels, bytes_per_sample, n_frames;
+    void* host_out_px;        /* nullable: copy the image to host memory when done */
+    uint8_t* host_out_mask;   /* nullable */
+} cp_request;
+
+typedef struct {
+    uint64_t submitted, completed, rejected;
+    uint64_t pixel_samples;   /* frames * out_h * out_w over completed requests */
+    uint64_t peak_bytes;      /* highest admitted HBM use */
+    uint32_t peak_inflight;
+} cp_server_stats;
+
 CP_API int cp_server_create(cp_ctx* ctx, uint64_t hbm_budget_bytes, uint32_t max_inflight,
                             cp_server** out);
-CP_API int cp_server_submit(cp_server* srv, const cp_job* job, uint64_t* ticket);
+/* Queue a request; the caller's arrays must stay valid until the request completes. */
+CP_API int cp_server_submit(cp_server* srv, const cp_request* req, uint64_t* ticket);
 CP_API int cp_server_poll(cp_server* srv, uint64_t ticket, int32_t* done);
+/* Block until the request is done; latency_ms = submit to completion.  Returns the
+ * request's own status. */
 CP_API int cp_server_wait(cp_server* srv, uint64_t ticket, double* latency_ms);
+CP_API int cp_server_stats_get(cp_server* srv, cp_server_stats* out);
+/* Drains in-flight work, then stops the worker. */
 CP_API int cp_server_destroy(cp_server* srv);
 
 #ifdef __cplusplus
