@@ -54,6 +54,156 @@ WORKLOADS = {
 }
 
 
+def _stack_from_scene(cpa, ctx, sc):
+    st = cpa.Stack(ctx, sc.width, sc.height, sc.channels, sc.bytes_per_sample, capacity=sc.n_frames)
+    sc.generate_device(st.device_ptr, 0, sc.n_frames, stream=ctx.stream)
+    ctx.synchronize()
+    st.commit(sc.n_frames)
+    return st
+
+
+def run_batching(cpa, H, ctx, seed=3, n_stacks=48):
+    """48 independent RGB8 stacks (frames U{8..200}, 640x480..1920x1080, abs/rel thresholds
+    alternating, the six --outlier modes round-robin): one persistent graph launch vs one
+    launch per stack."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    jobs, bufs, total = [], [], 0
+    sels = list(cpa.OutlierSelectionMode)
+    for i in range(n_stacks):
+        n = int(rng.integers(8, 201))
+        w = int(rng.integers(640, 1921))
+        h = int(rng.integers(480, 1081))
+        sc = cpa.Scene(seed=1000 + i, width=w, height=h, n_frames=n, n_disks=2, disk_r_min=10,
+                       disk_r_max=30, blob_permille=100)
+        st = _stack_from_scene(cpa, ctx, sc)
+        thr = cpa.Threshold.abs(0.05, 0.2) if i % 2 == 0 else cpa.Threshold.rel(2.0, 4.0)
+        proc = cpa.OutlierProcessor(thr, cpa.BackgroundMode.FIRST, sels[i % 6])
+        px, mask = H.DeviceBuffer(w * h * 3), H.DeviceBuffer(w * h * 3)
+        jobs.append(cpa.Job(cpa.CP_MODE_OUTLIER, st, px.ptr, mask.ptr, processor=proc))
+        bufs.append((st, px, mask))
+        total += n * w * h
+    lib = ctx.lib
+
+    def plain():
+        for j in jobs:
+            outs = cpa.cp_outputs(j.c.out.px, j.c.out.mask, None, None)
+            cpa._check(lib.cp_outlier(ctx.handle, j.stack.handle, C.byref(j.sel.c), C.byref(j.c.outlier),
+                                      C.byref(outs), cpa.CP_FLAG_DEVICE_OUT))
+            ctx.synchronize()  # one stack per launch, each waited for like a drop-in call
+
+    def timed(fn, reps=3):
+        fn()
+        ctx.synchronize()
+        best = 1e30
+        for _ in range(reps):
+            e0, e1 = H.Event(), H.Event()
+            t0 = time.perf_counter()
+            e0.record(ctx.stream)
+            fn()
+            e1.record(ctx.stream)
+            e1.synchronize()
+            best = min(best, time.perf_counter() - t0)
+        return best
+
+    plain_s = timed(plain)
+    ref = [b[1].to_host(np.zeros(b[1].nbytes, np.uint8)) for b in bufs]
+    batch = cpa.Batch(ctx, jobs, graph=True)
+    batch_s = timed(lambda: batch.launch())
+    same = all(np.array_equal(b[1].to_host(np.zeros(b[1].nbytes, np.uint8)), r) for b, r in zip(bufs, ref))
+    batch.close()
+    for st, px, mask in bufs:
+        st.close()
+        px.free()
+        mask.free()
+    return {"stacks": n_stacks, "pixel_samples": total,
+            "plain_value": total / plain_s, "batched_value": total / batch_s,
+            "speedup": plain_s / batch_s, "byte_identical": bool(same),
+            "plain": "48 sequential cp_outlier launches, one stack each, each waited for",
+            "batched": "one persistent graph launch (cp_batch_create/cp_batch_launch)"}
+
+
+def run_serving(cpa, H, ctx, seed=4, n_jobs=24, window=60, n_video=600):
+    """A 600-frame RGB8 1080p chrono-video (60-frame windows, stride 1 -> 540 windows over
+    device-resident frames) interleaved with Poisson-arriving single-image jobs of 16-128
+    frames at 720p/1080p uploaded from pinned host memory; served (continuous batching,
+    HBM-budget admission) vs plain (one request at a time, image returned to the host)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    vsc = cpa.Scene(seed=seed, width=1920, height=1080, n_frames=n_video, n_disks=3, disk_r_min=10,
+                    disk_r_max=40, blob_permille=100)
+    video = _stack_from_scene(cpa, ctx, vsc)
+    wins = [ix for _, ix in cpa.video_windows(cpa.FrameRange(0, window, 1),
+                                               cpa.FrameRange(0, n_video - window, 1), n_video)]
+    proc = cpa.OutlierProcessor(cpa.Threshold.abs(0.05, 0.2), cpa.BackgroundMode.FIRST,
+                                cpa.OutlierSelectionMode.EXTREME)
+    # host frame pools for the single-image jobs (pinned, copied from device-generated frames)
+    pools = {}
+    for (w, h) in ((1280, 720), (1920, 1080)):
+        sc = cpa.Scene(seed=seed + w, width=w, height=h, n_frames=128, n_disks=2, disk_r_min=10,
+                       disk_r_max=30, blob_permille=100)
+        st = _stack_from_scene(cpa, ctx, sc)
+        pools[(w, h)] = H.PinnedArray((128, h, w, 3), np.uint8)
+        st_arr = st.download()
+        pools[(w, h)].array[...] = st_arr
+        st.close()
+    jobs_spec = []
+    t = 0.0
+    for i in range(n_jobs):
+        t += rng.exponential(1.0 / n_jobs)  # arrival times over the run, in units of its length
+        n = int(rng.integers(16, 129))
+        wh = (1280, 720) if rng.random() < 0.5 else (1920, 1080)
+        jobs_spec.append((t, n, wh))
+    total = sum(len(ix) for ix in wins) * 1920 * 1080 + sum(n * wh[0] * wh[1] for _, n, wh in jobs_spec)
+    outs_w = [H.PinnedArray((1080, 1920, 3), np.uint8) for _ in range(4)]
+    # ---- plain: one request at a time, image back on the host
+    lib = ctx.lib
+    t0 = time.perf_counter()
+    ji = 0
+    for k, ix in enumerate(wins):
+        img, _ = proc.process(video, image_indices=ix, with_mask=False)
+        while ji < len(jobs_spec) and jobs_spec[ji][0] * len(wins) <= k:
+            _, n, (w, h) = jobs_spec[ji]
+            st = cpa.Stack.from_array(ctx, pools[(w, h)].array[:n])
+            proc.process(st, with_mask=False)
+            st.close()
+            ji += 1
+    plain_s = time.perf_counter() - t0
+    # ---- served
+    srv = cpa.Server(ctx, hbm_budget=16 << 30, max_inflight=8)
+    tickets, kinds = [], []
+    job_outs = [H.PinnedArray((1080, 1920, 3), np.uint8) for _ in range(len(jobs_spec))]
+    t0 = time.perf_counter()
+    ji = 0
+    for k, ix in enumerate(wins):
+        job = cpa.Job(cpa.CP_MODE_OUTLIER, video, None, processor=proc, image_indices=ix)
+        tickets.append(srv.submit(job, host_out=outs_w[k % 4].array))
+        kinds.append("window")
+        while ji < len(jobs_spec) and jobs_spec[ji][0] * len(wins) <= k:
+            _, n, (w, h) = jobs_spec[ji]
+            job = cpa.Job(cpa.CP_MODE_OUTLIER, None, None, processor=proc, n_frames=n, bytes_per_sample=1)
+            tickets.append(srv.submit(job, host_frames=pools[(w, h)].array[:n],
+                                      host_out=job_outs[ji].array[:h, :w]))
+            kinds.append("job")
+            ji += 1
+    lat = [srv.wait(tk) for tk in tickets]
+    served_s = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    video.close()
+    lat = np.array(lat)
+    jl = np.array([l for l, k in zip(lat, kinds) if k == "job"])
+    return {"windows": len(wins), "window_frames": window, "jobs": len(jobs_spec),
+            "pixel_samples": int(total), "plain_value": total / plain_s, "served_value": total / served_s,
+            "speedup": plain_s / served_s, "p50_ms": float(np.percentile(lat, 50)),
+            "p99_ms": float(np.percentile(lat, 99)),
+            "job_p50_ms": float(np.percentile(jl, 50)) if len(jl) else None,
+            "job_p99_ms": float(np.percentile(jl, 99)) if len(jl) else None,
+            "peak_inflight": st["peak_inflight"], "peak_hbm_bytes": st["peak_bytes"],
+            "plain": "one request at a time (cp_outlier, image copied to the host)",
+            "served": "cp_server: 8 stream slots, 16 GiB budget, continuous admission"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -63,6 +213,7 @@ def main():
     ap.add_argument("--cpu-rows", type=int, default=256, help="output rows of the CPU baseline band")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--force-generic", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="skip the batching and serving measurements")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (scale factor for HBM byte counters)")
     args = ap.parse_args()
@@ -199,6 +350,19 @@ def main():
         print(f"[bench] cpu oracle {cpu['value']:.4g} px-samples/s on {threads} threads, "
               f"{cpu['single_thread_value']:.4g} on 1 thread; GPU rows bit-exact: {match}", file=sys.stderr)
 
+    if not args.no_extras and args.workload == "outlier_4k_rgb16_256f":
+        stack.close()
+        out_px.free()
+        out_mask.free()
+        t0 = time.time()
+        extra["batching"] = run_batching(cpa, H, ctx)
+        print(f"[bench] batching: {extra['batching']['speedup']:.2f}x over plain "
+              f"({time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
+        t0 = time.time()
+        extra["serving"] = run_serving(cpa, H, ctx)
+        print(f"[bench] serving: {extra['serving']['speedup']:.2f}x over plain, p50 "
+              f"{extra['serving']['p50_ms']:.2f} ms p99 {extra['serving']['p99_ms']:.2f} ms "
+              f"({time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
     res = {
         "metric": "pixel-samples/s outlier mode (% HBM roofline)" if wl["mode"] == "outlier"
         else f"pixel-samples/s {wl['mode']} mode (% HBM roofline)",
@@ -217,7 +381,6 @@ def main():
         **extra,
     }
     print(json.dumps(res))
-    stack.close()
     ctx.close()
 
 

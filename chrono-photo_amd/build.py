@@ -2,6 +2,7 @@
 
 libchrono_hip.so   -- the compositing library behind include/chrono_hip.h
 libchrono_synth.so -- seeded synthetic stacks (tests and benchmarks)
+chrono-photo-hip   -- the command-line tool (cli/), linked against libchrono_hip.so
 
 Every kernel translation unit is compiled with the flags that keep the reference's f32
 semantics: no FMA contraction, correctly rounded division and square root, no denormal
@@ -14,6 +15,7 @@ import concurrent.futures as cf
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
+CLI = os.path.join(HERE, "cli")
 OUT = os.path.join(HERE, "lib")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
@@ -60,9 +62,29 @@ def build(verbose=False):
         os.replace(out + ".tmp", out)
         if verbose:
             print("built", out)
+    build_cli(verbose)
     return OUT
 
 
+def build_cli(verbose=False):
+    """Host-only C++ (g++): the CLI reaches the GPU through the C ABI alone."""
+    out = os.path.join(OUT, "chrono-photo-hip")
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-o", out + ".tmp",
+           os.path.join(CLI, "main.cpp"), os.path.join(CLI, "imageio.cpp"),
+           "-L" + OUT, "-lchrono_hip", "-lz", "-Wl,-rpath,$ORIGIN",
+           "-Wl,-rpath-link,/opt/rocm/lib"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"CLI build failed: {' '.join(cmd)}\n{r.stdout}{r.stderr}")
+    os.replace(out + ".tmp", out)
+    if verbose:
+        print("built", out)
+    return out
+
+
 if __name__ == "__main__":
-    build(verbose=True)
+    if sys.argv[1:] == ["cli"]:
+        build_cli(verbose=True)
+    else:
+        build(verbose=True)
     sys.exit(0)

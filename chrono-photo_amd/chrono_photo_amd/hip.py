@@ -37,6 +37,8 @@ def hip() -> C.CDLL:
         h.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
         h.hipEventDestroy.argtypes = [C.c_void_p]
         h.hipDeviceSynchronize.argtypes = []
+        h.hipHostMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t, C.c_uint]
+        h.hipHostFree.argtypes = [C.c_void_p]
         h.hipGetErrorString.restype = C.c_char_p
         h.hipGetErrorString.argtypes = [C.c_int]
     return _hip
@@ -98,3 +100,22 @@ def copy_rows_to_host(out, dev_ptr: int, frame_bytes: int, n_frames: int):
     check(hip().hipMemcpy2D(out.ctypes.data, width, dev_ptr, frame_bytes, width, n_frames,
                             hipMemcpyDeviceToHost), "hipMemcpy2D")
     return out
+
+
+class PinnedArray:
+    """numpy view of page-locked host memory (hipHostMalloc), for asynchronous copies."""
+
+    def __init__(self, shape, dtype):
+        import numpy as np
+        nbytes = int(np.prod(shape)) * np.dtype(dtype).itemsize
+        p = C.c_void_p()
+        check(hip().hipHostMalloc(C.byref(p), max(nbytes, 1), 0), "hipHostMalloc")
+        self.ptr = p.value
+        buf = (C.c_uint8 * nbytes).from_address(self.ptr)
+        self.array = np.frombuffer(buf, dtype=dtype).reshape(shape)
+
+    def free(self):
+        if self.ptr:
+            self.array = None
+            hip().hipHostFree(self.ptr)
+            self.ptr = None

@@ -1,0 +1,232 @@
+"""The command-line tool (chrono-photo_amd/cli): image codec and option handling on the CPU,
+end-to-end composites against the oracle on the GPU.  The flow mirrors src/main.rs:21-98."""
+import os
+import struct
+import subprocess
+import zlib
+
+import numpy as np
+import pytest
+
+import chrono_photo_amd as cpa
+import oracle_py as orc
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CLI = os.path.join(ROOT, "chrono-photo_amd", "lib", "chrono-photo-hip")
+
+
+# ------------------------------------------------------------------ PNG/PPM helpers
+def _paeth(a, b, c):
+    p = a + b - c
+    pa, pb, pc = abs(p - a), abs(p - b), abs(p - c)
+    return np.where((pa <= pb) & (pa <= pc), a, np.where(pb <= pc, b, c))
+
+
+def write_png(path, arr, filters=(0, 1, 2, 3, 4)):
+    """PNG writer that cycles through every filter type (row i uses filters[i % len])."""
+    arr = np.asarray(arr)
+    if arr.ndim == 2:
+        arr = arr[:, :, None]
+    h, w, c = arr.shape
+    depth = 16 if arr.dtype == np.uint16 else 8
+    ctype = {1: 0, 2: 4, 3: 2, 4: 6}[c]
+    rows = arr.astype(">u2").view(np.uint8) if depth == 16 else arr
+    rows = rows.reshape(h, -1).astype(np.int32)
+    bpp = c * depth // 8
+    raw = bytearray()
+    prev = np.zeros(rows.shape[1], np.int32)
+    for y in range(h):
+        ft = filters[y % len(filters)]
+        x = rows[y]
+        a = np.concatenate([np.zeros(bpp, np.int32), x[:-bpp]])
+        cc = np.concatenate([np.zeros(bpp, np.int32), prev[:-bpp]])
+        pred = [np.zeros_like(x), a, prev, (a + prev) // 2, _paeth(a, prev, cc)][ft]
+        raw.append(ft)
+        raw += ((x - pred) & 0xFF).astype(np.uint8).tobytes()
+        prev = x
+    def chunk(t, body):
+        return struct.pack(">I", len(body)) + t + body + struct.pack(">I", zlib.crc32(t + body) & 0xFFFFFFFF)
+    ihdr = struct.pack(">IIBBBBB", w, h, depth, ctype, 0, 0, 0)
+    with open(path, "wb") as f:
+        f.write(b"\x89PNG\r\n\x1a\n" + chunk(b"IHDR", ihdr) + chunk(b"IDAT", zlib.compress(bytes(raw)))
+                + chunk(b"IEND", b""))
+
+
+def read_png(path):
+    """Decoder for the PNGs the tool writes (filter 0 rows, one IDAT)."""
+    data = open(path, "rb").read()
+    assert data[:8] == b"\x89PNG\r\n\x1a\n"
+    pos, idat = 8, b""
+    while pos < len(data):
+        n, t = struct.unpack(">I4s", data[pos:pos + 8])
+        body = data[pos + 8:pos + 8 + n]
+        if t == b"IHDR":
+            w, h, depth, ctype = struct.unpack(">IIBB", body[:10])
+        elif t == b"IDAT":
+            idat += body
+        assert struct.unpack(">I", data[pos + 8 + n:pos + 12 + n])[0] == zlib.crc32(t + body) & 0xFFFFFFFF
+        pos += 12 + n
+    c = {0: 1, 4: 2, 2: 3, 6: 4}[ctype]
+    raw = np.frombuffer(zlib.decompress(idat), np.uint8).reshape(h, -1)
+    assert (raw[:, 0] == 0).all()
+    px = raw[:, 1:]
+    if depth == 16:
+        return px.copy().view(">u2").astype(np.uint16).reshape(h, w, c)
+    return px.reshape(h, w, c).copy()
+
+
+def read_pnm(path):
+    data = open(path, "rb").read()
+    parts = data.split(maxsplit=4)
+    magic, w, h, maxv = parts[0], int(parts[1]), int(parts[2]), int(parts[3])
+    c = 3 if magic == b"P6" else 1
+    dt = np.dtype(">u2") if maxv > 255 else np.dtype(np.uint8)
+    px = np.frombuffer(parts[4], dt, count=w * h * c)
+    return px.astype(np.uint16 if maxv > 255 else np.uint8).reshape(h, w, c)
+
+
+@pytest.fixture(scope="module")
+def codec(tmp_path_factory):
+    d = tmp_path_factory.mktemp("codec")
+    exe = str(d / "imageio_roundtrip")
+    src = os.path.join(ROOT, "tests", "native", "imageio_roundtrip.cpp")
+    imp = os.path.join(ROOT, "chrono-photo_amd", "cli", "imageio.cpp")
+    subprocess.run(["g++", "-O1", "-std=c++17", "-o", exe, src, imp, "-lz"], check=True)
+
+    def run(src_path, dst_path):
+        r = subprocess.run([exe, src_path, dst_path], capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr
+        return tuple(int(v) for v in r.stdout.split())
+    return run
+
+
+# ------------------------------------------------------------------ CPU: codec
+@pytest.mark.parametrize("channels", [1, 2, 3, 4])
+@pytest.mark.parametrize("dtype", [np.uint8, np.uint16])
+def test_png_decode_all_filters(codec, tmp_path, channels, dtype):
+    rng = np.random.default_rng(channels)
+    hi = 256 if dtype == np.uint8 else 65536
+    arr = rng.integers(0, hi, (23, 17, channels)).astype(dtype)
+    arr[5:9] = arr[4]  # runs that make the Up/Paeth predictors exact
+    src = str(tmp_path / "in.png")
+    write_png(src, arr)
+    dst = str(tmp_path / "out.png")
+    assert codec(src, dst) == (17, 23, channels, np.dtype(dtype).itemsize)
+    np.testing.assert_array_equal(read_png(dst), arr)
+
+
+@pytest.mark.parametrize("mode", ["L", "RGB", "RGBA"])
+def test_png_decode_pil_written(codec, tmp_path, mode):
+    Image = pytest.importorskip("PIL.Image")
+    rng = np.random.default_rng(7)
+    c = {"L": 1, "RGB": 3, "RGBA": 4}[mode]
+    arr = rng.integers(0, 256, (31, 29, c), dtype=np.uint8)
+    src = str(tmp_path / "pil.png")
+    Image.fromarray(arr[:, :, 0] if c == 1 else arr, mode).save(src, optimize=True)
+    dst = str(tmp_path / "back.png")
+    codec(src, dst)
+    back = np.asarray(Image.open(dst))
+    np.testing.assert_array_equal(back.reshape(arr.shape), arr)
+
+
+@pytest.mark.parametrize("channels", [1, 3])
+@pytest.mark.parametrize("dtype", [np.uint8, np.uint16])
+def test_pnm_roundtrip(codec, tmp_path, channels, dtype):
+    rng = np.random.default_rng(11)
+    hi = 256 if dtype == np.uint8 else 65536
+    arr = rng.integers(0, hi, (9, 14, channels)).astype(dtype)
+    src = str(tmp_path / "in.png")
+    write_png(src, arr)
+    dst = str(tmp_path / ("out.ppm" if channels == 3 else "out.pgm"))
+    codec(src, dst)
+    np.testing.assert_array_equal(read_pnm(dst), arr)
+    dst2 = str(tmp_path / "again.png")
+    codec(dst, dst2)
+    np.testing.assert_array_equal(read_png(dst2), arr)
+
+
+def test_codec_rejects_jpeg(codec, tmp_path):
+    p = tmp_path / "x.jpg"
+    p.write_bytes(b"\xff\xd8\xff\xe0" + b"\0" * 32)
+    with pytest.raises(AssertionError):
+        codec(str(p), str(tmp_path / "y.png"))
+
+
+# ------------------------------------------------------------------ CPU: options
+def _cli(*args, cwd=None):
+    if not os.path.exists(CLI):
+        pytest.skip("CLI not built")
+    return subprocess.run([CLI, *args], capture_output=True, text=True, cwd=cwd, timeout=120)
+
+
+def test_cli_option_errors(tmp_path):
+    r = _cli("--pattern", "x*.png")
+    assert r.returncode != 0 and "--output" in r.stderr
+    r = _cli("--pattern", "x*.png", "--output", "o.png", "--shake", "10/5")
+    assert r.returncode != 0 and "--shake-anchors" in r.stderr
+    r = _cli("--pattern", "x*.png", "--output", "o.png", "--frames", "1/2")
+    assert r.returncode != 0 and "3 elements" in r.stderr
+    r = _cli("--pattern", "x*.png", "--output", "o.png", "--bogus")
+    assert r.returncode != 0 and "unknown option" in r.stderr
+    r = _cli("--pattern", str(tmp_path / "none*.png"), "--output", "o.png")
+    assert r.returncode != 0 and "No files found" in r.stderr
+
+
+# ------------------------------------------------------------------ GPU: end to end
+# Background median and first are deterministic; the tool's random background uses the
+# library's seeded draw (seed 0 unless --seed is given), which the oracle shares.
+def _frames(tmp_path, n=12, w=48, h=40, c=3, bps=1, seed=5):
+    sc = cpa.Scene(seed=seed, width=w, height=h, channels=c, bytes_per_sample=bps, n_frames=n,
+                   n_disks=2, disk_r_min=5, disk_r_max=8)
+    st = sc.generate()
+    for i in range(n):
+        write_png(str(tmp_path / f"frame-{i:03d}.png"), st[i])
+    return st, str(tmp_path / "frame-*.png")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("bps", [1, 2])
+def test_cli_outlier_image(tmp_path, bps):
+    st, pat = _frames(tmp_path, bps=bps)
+    out = str(tmp_path / "out.png")
+    blend = str(tmp_path / "blend.png")
+    r = _cli("--pattern", pat, "--output", out, "--output-blend", blend, "--mode", "outlier",
+             "--threshold", "abs/0.05/0.2", "--background", "median", "--outlier", "extreme",
+             "--weights", "1", "0.5", "2", "1")
+    assert r.returncode == 0, r.stderr
+    img, mask, _ = orc.outlier(st, orc.threshold(True, 0.05, 0.2, bps), 3, 2,
+                               weights=(1, 0.5, 2, 1))
+    np.testing.assert_array_equal(read_png(out), img)
+    np.testing.assert_array_equal(read_png(blend), mask)
+
+
+@pytest.mark.gpu
+def test_cli_options_file_and_frames(tmp_path):
+    st, pat = _frames(tmp_path)
+    out = str(tmp_path / "o.png")
+    opts = tmp_path / "run.chrono"
+    opts.write_text(f'--pattern "{pat}"\n--output {out}\n--frames 2/11/2\n--mode lighter\n'
+                    '--fade clamp/abs/0,0.5/4,1.0\n')
+    r = _cli(str(opts))
+    assert r.returncode == 0, r.stderr
+    sel = np.arange(2, 11, 2)
+    ref = orc.simple(st[sel], fade=orc.Fade(False, True, [0, 4], [0.5, 1.0]), darker=False)
+    np.testing.assert_array_equal(read_png(out), ref)
+
+
+@pytest.mark.gpu
+def test_cli_video_windows(tmp_path):
+    st, pat = _frames(tmp_path, n=10)
+    out = str(tmp_path / "vid.png")
+    r = _cli("--pattern", pat, "--output", out, "--video-in", "-2/2/1", "--video-out", "./8/2",
+             "--background", "first", "--threshold", "abs/0.02/0.1")
+    assert r.returncode == 0, r.stderr
+    wins = orc.video_windows((-2, 2, 1), (None, 8, 2), 10)
+    assert len(wins) > 0
+    thr = orc.threshold(True, 0.02, 0.1, 1)
+    for k, slots in wins:
+        if not slots:
+            assert not os.path.exists(str(tmp_path / f"vid-{k:05d}.png"))
+            continue
+        img, _, _ = orc.outlier(st, thr, 0, 2, slots=slots)
+        np.testing.assert_array_equal(read_png(str(tmp_path / f"vid-{k:05d}.png")), img, err_msg=f"window {k}")
