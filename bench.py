@@ -172,7 +172,7 @@ def run_serving(cpa, H, ctx, seed=4, n_jobs=24, window=60, n_video=600):
     # ---- served
     srv = cpa.Server(ctx, hbm_budget=16 << 30, max_inflight=8)
     tickets, kinds = [], []
-    job_outs = [H.PinnedArray((1080, 1920, 3), np.uint8) for _ in range(len(jobs_spec))]
+    job_outs = [H.PinnedArray((h, w, 3), np.uint8) for _, _, (w, h) in jobs_spec]
     t0 = time.perf_counter()
     ji = 0
     for k, ix in enumerate(wins):
@@ -183,7 +183,7 @@ def run_serving(cpa, H, ctx, seed=4, n_jobs=24, window=60, n_video=600):
             _, n, (w, h) = jobs_spec[ji]
             job = cpa.Job(cpa.CP_MODE_OUTLIER, None, None, processor=proc, n_frames=n, bytes_per_sample=1)
             tickets.append(srv.submit(job, host_frames=pools[(w, h)].array[:n],
-                                      host_out=job_outs[ji].array[:h, :w]))
+                                      host_out=job_outs[ji].array))
             kinds.append("job")
             ji += 1
     lat = [srv.wait(tk) for tk in tickets]

@@ -870,6 +870,9 @@ class Server:
             r.width, r.height, r.channels = ww, hh, cc
             r.bytes_per_sample = host_frames.dtype.itemsize
             r.n_frames = n
+        for a in (host_out, host_mask):
+            if a is not None and not a.flags.c_contiguous:
+                raise ValueError("host_out / host_mask must be C-contiguous [h][w][c] arrays")
         r.host_out_px = None if host_out is None else host_out.ctypes.data
         r.host_out_mask = None if host_mask is None else host_mask.ctypes.data
         t = C.c_uint64()

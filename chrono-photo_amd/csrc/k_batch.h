@@ -21,9 +21,29 @@ struct BatchJobK {
 };
 
 template <int B, int C, int SH, bool ABS>
-__device__ __forceinline__ void batch_tile(const OutlierK& k, uint32_t bx, uint32_t y,
-                                           const uint64_t* s_off) {
-    outlier_tile<B, C, kShapes[SH].G, kShapes[SH].F, ABS>(k, bx, y, s_off);
+__device__ __forceinline__ void batch_tile(const OutlierK& k, uint32_t bx, uint32_t y, unsigned char* smem) {
+    constexpr int G = kShapes[SH].G, F = kShapes[SH].F;
+    outlier_tile<B, C, G, F, ABS>(k, bx, y, *reinterpret_cast<TileShared<C, G, F>*>(smem));
+}
+
+// Frame positions covered by a tile of shape sh (G * F).
+__device__ __forceinline__ uint32_t shape_frames(int sh) {
+    constexpr uint32_t gf[kBatchShapeMax + 1] = {
+        kShapes[0].G * kShapes[0].F, kShapes[1].G * kShapes[1].F, kShapes[2].G * kShapes[2].F,
+        kShapes[3].G * kShapes[3].F, kShapes[4].G * kShapes[4].F, kShapes[5].G * kShapes[5].F,
+        kShapes[6].G * kShapes[6].F, kShapes[7].G * kShapes[7].F};
+    return gf[sh];
+}
+
+// LDS of the largest tile shape the batch kernel carries.
+template <int C, int SH>
+constexpr size_t batch_smem() {
+    if constexpr (SH < 0) return 0;
+    else {
+        constexpr size_t a = sizeof(TileShared<C, kShapes[SH].G, kShapes[SH].F>);
+        constexpr size_t b = batch_smem<C, SH - 1>();
+        return a > b ? a : b;
+    }
 }
 
 template <int B>
@@ -33,7 +53,9 @@ template <int B, int C>
 __global__ void __launch_bounds__(256) outlier_batch_kernel(const BatchJobK* __restrict__ jobs,
                                                             uint32_t njobs, uint32_t total,
                                                             unsigned int* __restrict__ counter) {
-    __shared__ uint64_t s_off[256];
+    // every TileShared starts with its frame-offset table
+    __shared__ __align__(16) unsigned char smem[batch_smem<C, kBatchShapeMax>()];
+    uint64_t* const s_off = reinterpret_cast<uint64_t*>(smem);
     __shared__ uint32_t s_item;
     uint32_t cur = 0xFFFFFFFFu;
     for (;;) {
@@ -49,31 +71,32 @@ __global__ void __launch_bounds__(256) outlier_batch_kernel(const BatchJobK* __r
         }
         const BatchJobK& J = jobs[lo];
         if (lo != cur) {
-            for (uint32_t i = threadIdx.x; i < J.k.n; i += blockDim.x) s_off[i] = J.k.pos_off[i];
+            const uint32_t nf = shape_frames(J.shape);
+            load_offsets<B>(s_off, nf, J.k.pos_off, J.k.n);
             cur = lo;
         }
         __syncthreads();
         const uint32_t tile = item - J.item0;
         const uint32_t bx = tile % J.tiles_x, y = tile / J.tiles_x;
         switch (J.shape * 2 + (J.k.th.absolute ? 1 : 0)) {
-        case 0: batch_tile<B, C, 0, false>(J.k, bx, y, s_off); break;
-        case 1: batch_tile<B, C, 0, true>(J.k, bx, y, s_off); break;
-        case 2: batch_tile<B, C, 1, false>(J.k, bx, y, s_off); break;
-        case 3: batch_tile<B, C, 1, true>(J.k, bx, y, s_off); break;
-        case 4: batch_tile<B, C, 2, false>(J.k, bx, y, s_off); break;
-        case 5: batch_tile<B, C, 2, true>(J.k, bx, y, s_off); break;
-        case 6: batch_tile<B, C, 3, false>(J.k, bx, y, s_off); break;
-        case 7: batch_tile<B, C, 3, true>(J.k, bx, y, s_off); break;
-        case 8: batch_tile<B, C, 4, false>(J.k, bx, y, s_off); break;
-        case 9: batch_tile<B, C, 4, true>(J.k, bx, y, s_off); break;
-        case 10: batch_tile<B, C, 5, false>(J.k, bx, y, s_off); break;
-        case 11: batch_tile<B, C, 5, true>(J.k, bx, y, s_off); break;
-        case 12: batch_tile<B, C, 6, false>(J.k, bx, y, s_off); break;
-        case 13: batch_tile<B, C, 6, true>(J.k, bx, y, s_off); break;
-        case 14: batch_tile<B, C, 7, false>(J.k, bx, y, s_off); break;
-        default: batch_tile<B, C, 7, true>(J.k, bx, y, s_off); break;
+        case 0: batch_tile<B, C, 0, false>(J.k, bx, y, smem); break;
+        case 1: batch_tile<B, C, 0, true>(J.k, bx, y, smem); break;
+        case 2: batch_tile<B, C, 1, false>(J.k, bx, y, smem); break;
+        case 3: batch_tile<B, C, 1, true>(J.k, bx, y, smem); break;
+        case 4: batch_tile<B, C, 2, false>(J.k, bx, y, smem); break;
+        case 5: batch_tile<B, C, 2, true>(J.k, bx, y, smem); break;
+        case 6: batch_tile<B, C, 3, false>(J.k, bx, y, smem); break;
+        case 7: batch_tile<B, C, 3, true>(J.k, bx, y, smem); break;
+        case 8: batch_tile<B, C, 4, false>(J.k, bx, y, smem); break;
+        case 9: batch_tile<B, C, 4, true>(J.k, bx, y, smem); break;
+        case 10: batch_tile<B, C, 5, false>(J.k, bx, y, smem); break;
+        case 11: batch_tile<B, C, 5, true>(J.k, bx, y, smem); break;
+        case 12: batch_tile<B, C, 6, false>(J.k, bx, y, smem); break;
+        case 13: batch_tile<B, C, 6, true>(J.k, bx, y, smem); break;
+        case 14: batch_tile<B, C, 7, false>(J.k, bx, y, smem); break;
+        default: batch_tile<B, C, 7, true>(J.k, bx, y, smem); break;
         }
-        __syncthreads();  // s_off and s_item are rewritten next iteration
+        __syncthreads();  // the tile's LDS and s_item are rewritten next iteration
     }
 }
 

@@ -4,28 +4,29 @@
 // The G lanes of a pixel (adjacent lanes, one DPP quad or half-row) each hold a contiguous
 // block of F frames x C channels in VGPRs, two samples per VGPR (16-bit halves, frames
 // 2j and 2j+1), so the stack tile is read from HBM exactly once and stays on chip for the
-// order-statistic pass, the classification pass and the blend.  Frame t's element offset
-// (stack slot + crop origin) comes from a per-workgroup LDS table.
+// order-statistic pass and the classification pass.  Frame t's element offset (stack slot
+// + crop origin) comes from a per-workgroup LDS table padded to G*F entries.
 //
 // Order statistics (src/chrono.rs:228-255 sorts each weighted channel; any exact selection
 // gives the same values).  The r-th smallest value is found MSB-first: keep the largest
 // prefix v with #{y < v} <= r.  Counts come from the sum of absolute differences:
 //     sum_y |y - v| - sum_y |y - (v-1)| = 2 #{y < v} - n
 // so one search step costs two v_sad_u8 per four byte-samples plus a G-lane DPP reduction.
-// 8-bit stacks need one 8-step level; 16-bit stacks run one level on the high bytes and
-// one on the low bytes clamped into the winning high-byte bucket (values below map to 0,
-// above to 255, which leaves every count #{y < v}, v in [1,255], exact).  The neighbour
-// order statistic of an even-sized median (and the quartile interpolation partners) is
-// either the same value (when #{x <= v} covers it) or min{x > v} (packed u16 min).
-// Frames beyond N are padded with the maximum sample value, which never changes the ranks
-// below N.
+// All channels (and, for relative thresholds, all ranks) advance through the same step, so
+// the independent chains interleave.  8-bit stacks need one 8-step level; 16-bit stacks run
+// one level on the high bytes and one on the low bytes clamped into the winning high-byte
+// bucket (values below map to 0, above to 255, which leaves every count #{y < v}, v in
+// [1,255], exact).  The neighbour order statistic of an even-sized median (and the quartile
+// interpolation partners) is either the same value (when #{x <= v} covers it) or min{x > v}
+// (packed u16 min).  Frames beyond N are padded with the maximum sample value, which never
+// changes the ranks below N.
 //
-// Classification (src/chrono.rs:257-288) runs over the lane's frames in order with the
-// reference's f32 operation order; outliers become bits of a per-lane mask.  The group then
-// reduces counts, the first maximum (extreme mode) and exact integer sums (average modes;
-// the host routes stacks whose sums could exceed 2^24 to the generic kernel).  One lane per
-// pixel finishes: background and outlier selection / blending (src/chrono.rs:290-493),
-// re-reading from L2 only the few outlier samples an order-dependent blend needs.
+// Classification (src/chrono.rs:257-288) runs over the lane's frames with the reference's
+// f32 operation order; outliers become bits of a per-lane mask.  Each lane then leaves its
+// mask and its largest outlier distance in LDS, and after one barrier P lanes finish one
+// pixel each (background and outlier selection / blending, src/chrono.rs:290-493), so the
+// per-pixel tail runs on full waves instead of one lane in G.  The finish re-reads from L2
+// only the few samples an order-dependent blend needs.
 #pragma once
 #include <type_traits>
 
@@ -39,8 +40,11 @@ typedef unsigned short us2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ us2 as_us2(uint32_t v) { return __builtin_bit_cast(us2, v); }
 __device__ __forceinline__ uint32_t as_u32(us2 v) { return __builtin_bit_cast(uint32_t, v); }
 
+// Byte 0 of v in all four bytes.
+__device__ __forceinline__ uint32_t rep4(uint32_t v) { return __builtin_amdgcn_perm(0u, v, 0u); }
+
 // Sum / min / max over the G adjacent lanes of a pixel group (G in {1,2,4,8}); the DPP
-// moves fold into v_add/v_min/v_max_u32_dpp.
+// moves fold into v_add/v_min/v_max_u32_dpp.  Every lane of the group receives the result.
 template <int G>
 __device__ __forceinline__ int32_t grp_sum(int32_t v) {
     if constexpr (G >= 2) v += __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, true);   // xor 1
@@ -68,7 +72,7 @@ __device__ __forceinline__ uint32_t grp_max_u32(uint32_t v) {
 // #{bytes < v} over the group, v in [1, 255].
 template <int G, int ND>
 __device__ __forceinline__ uint32_t byte_count_lt(const uint32_t (&y)[ND], uint32_t v, int32_t ntot) {
-    const uint32_t V = v * 0x01010101u, Vm = V - 0x01010101u;
+    const uint32_t V = rep4(v), Vm = V - 0x01010101u;
     uint32_t a = 0, b = 0;
 #pragma unroll
     for (int i = 0; i < ND; ++i) {
@@ -78,47 +82,81 @@ __device__ __forceinline__ uint32_t byte_count_lt(const uint32_t (&y)[ND], uint3
     return (uint32_t)(ntot + grp_sum<G>((int32_t)(a - b))) >> 1;
 }
 
-// Largest v in [0, 255] with #{bytes < v} <= k over the group's ND dwords per lane.  The
-// candidate is kept replicated in all four bytes, so each step is two v_sad_u8 per dword,
-// a G-lane reduction and a select.
-template <int G, int ND>
-__device__ __forceinline__ uint32_t byte_select(const uint32_t (&y)[ND], uint32_t k, int32_t ntot) {
-    uint32_t lo = 0;
-    const int32_t two_k = 2 * (int32_t)k;
+// For every channel c and rank k[r]: the largest v in [0, 255] with #{bytes < v} <= k[r]
+// over the group's ND dwords per lane of y[c].  Candidates are kept replicated in all four
+// bytes, so each step is two v_sad_u8 per dword, a G-lane reduction and a select per
+// (channel, rank); the C*R chains are independent and interleave.
+template <int G, int ND, int C, int R>
+__device__ __forceinline__ void byte_select_multi(const uint32_t (&y)[C][ND], const uint32_t (&k)[R],
+                                                  int32_t ntot, uint32_t (&out)[C][R]) {
+    uint32_t lo[C][R];
+#pragma unroll
+    for (int c = 0; c < C; ++c)
+#pragma unroll
+        for (int r = 0; r < R; ++r) lo[c][r] = 0;
 #pragma unroll 1
     for (int bit = 7; bit >= 0; --bit) {
-        const uint32_t V = lo | (0x01010101u << bit);
-        const uint32_t Vm = V - 0x01010101u;
-        uint32_t a = 0, b = 0;
+        const uint32_t inc = 0x01010101u << bit;
 #pragma unroll
-        for (int i = 0; i < ND; ++i) {
-            a = __builtin_amdgcn_sad_u8(y[i], V, a);
-            b = __builtin_amdgcn_sad_u8(y[i], Vm, b);
+        for (int c = 0; c < C; ++c) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const uint32_t V = lo[c][r] | inc, Vm = V - 0x01010101u;
+                uint32_t a = 0, b = 0;
+#pragma unroll
+                for (int i = 0; i < ND; ++i) {
+                    a = __builtin_amdgcn_sad_u8(y[c][i], V, a);
+                    b = __builtin_amdgcn_sad_u8(y[c][i], Vm, b);
+                }
+                // ntot + d = 2 #{y < v}
+                if (ntot + grp_sum<G>((int32_t)(a - b)) <= 2 * (int32_t)k[r]) lo[c][r] = V;
+            }
         }
-        // ntot + d = 2 #{y < v}
-        if (ntot + grp_sum<G>((int32_t)(a - b)) <= two_k) lo = V;
     }
-    return lo & 0xFFu;
+#pragma unroll
+    for (int c = 0; c < C; ++c)
+#pragma unroll
+        for (int r = 0; r < R; ++r) out[c][r] = lo[c][r] & 0xFFu;
 }
 
-// Exact r-th smallest sample of one 16-bit channel by two byte levels.
-template <int G, int NW>
-__device__ __forceinline__ uint32_t select_rank_u16(const uint32_t (&r)[NW], uint32_t k, int32_t ntot) {
-    constexpr int ND = NW / 2;
-    uint32_t y[ND];
+// Low bytes of the 16-bit samples (8-bit stacks keep one sample per 16-bit half).
+template <int NW>
+__device__ __forceinline__ void pack_low_bytes(const uint32_t (&r)[NW], uint32_t (&y)[NW / 2]) {
 #pragma unroll
-    for (int i = 0; i < ND; ++i) y[i] = __builtin_amdgcn_perm(r[2 * i + 1], r[2 * i], 0x07050301u);
-    const uint32_t H = byte_select<G, ND>(y, k, ntot);
-    const us2 base = {(unsigned short)(H << 8), (unsigned short)(H << 8)};
+    for (int i = 0; i < NW / 2; ++i) y[i] = __builtin_amdgcn_perm(r[2 * i + 1], r[2 * i], 0x06040200u);
+}
+
+// clamp(x - base, 0, 255) of every 16-bit sample, packed four to a dword.
+template <int NW>
+__device__ __forceinline__ void pack_window(const uint32_t (&r)[NW], uint32_t base, uint32_t (&y)[NW / 2]) {
+    const us2 b2 = {(unsigned short)base, (unsigned short)base};
     const us2 c255 = {255, 255};
 #pragma unroll
-    for (int i = 0; i < ND; ++i) {
-        us2 a = __builtin_elementwise_min(__builtin_elementwise_sub_sat(as_us2(r[2 * i]), base), c255);
-        us2 b = __builtin_elementwise_min(__builtin_elementwise_sub_sat(as_us2(r[2 * i + 1]), base), c255);
+    for (int i = 0; i < NW / 2; ++i) {
+        us2 a = __builtin_elementwise_min(__builtin_elementwise_sub_sat(as_us2(r[2 * i]), b2), c255);
+        us2 b = __builtin_elementwise_min(__builtin_elementwise_sub_sat(as_us2(r[2 * i + 1]), b2), c255);
         y[i] = __builtin_amdgcn_perm(as_u32(b), as_u32(a), 0x06040200u);
     }
-    const uint32_t L = byte_select<G, ND>(y, k, ntot);
-    return (H << 8) | L;
+}
+
+// Exact r-th smallest of every channel of a 16-bit stack by two byte levels.
+template <int G, int NW, int C>
+__device__ __forceinline__ void select_rank_u16_multi(const uint32_t (&raw)[C][NW], uint32_t k, int32_t ntot,
+                                                      uint32_t (&v)[C]) {
+    constexpr int ND = NW / 2;
+    uint32_t y[C][ND];
+#pragma unroll
+    for (int c = 0; c < C; ++c)
+#pragma unroll
+        for (int i = 0; i < ND; ++i) y[c][i] = __builtin_amdgcn_perm(raw[c][2 * i + 1], raw[c][2 * i], 0x07050301u);
+    const uint32_t kk[1] = {k};
+    uint32_t H[C][1], L[C][1];
+    byte_select_multi<G, ND, C, 1>(y, kk, ntot, H);
+#pragma unroll
+    for (int c = 0; c < C; ++c) pack_window<NW>(raw[c], H[c][0] << 8, y[c]);
+    byte_select_multi<G, ND, C, 1>(y, kk, ntot, L);
+#pragma unroll
+    for (int c = 0; c < C; ++c) v[c] = (H[c][0] << 8) | L[c][0];
 }
 
 // #{x <= v} over the group (v below the maximum sample value).
@@ -176,66 +214,75 @@ __device__ __forceinline__ uint32_t group_estimate(const uint32_t (&r)[NW]) {
     return (s - grp_min_u32<G>(e) - grp_max_u32<G>(e)) / (uint32_t)(G - 2);
 }
 
-// Values at ranks (r0, r1), r1 in {r0, r0 + 1}, of one channel.
-//  u8:  one byte level; the neighbour comes from one more byte count.
-//  u16 (median): an 8-step byte search inside the window [lo, lo + 255] around the
-//       estimate; the result is exact when it ends strictly inside (1..254), otherwise
-//       the exact two-level search runs.  Group-uniform branches only.
-template <int B, int G, int NW, bool WINDOW>
-__device__ __forceinline__ void rank_pair(const uint32_t (&r)[NW], uint32_t r0, uint32_t r1, int32_t ntot,
-                                          uint32_t& v0, uint32_t& v1) {
-    constexpr int ND = NW / 2;
-    constexpr uint32_t MAXV = B == 1 ? 255u : 65535u;
-    uint32_t y[ND];
-    uint32_t cle = 0;
-    bool have_cle = false;
-    if constexpr (B == 1) {
-#pragma unroll
-        for (int i = 0; i < ND; ++i) y[i] = __builtin_amdgcn_perm(r[2 * i + 1], r[2 * i], 0x06040200u);
-        v0 = byte_select<G, ND>(y, r0, ntot);
-        if (r1 != r0) {
-            cle = v0 >= 255u ? (uint32_t)ntot : byte_count_lt<G, ND>(y, v0 + 1, ntot);
-            have_cle = true;
-        }
-    } else if constexpr (WINDOW) {
-        const uint32_t est = group_estimate<G, NW>(r);
-        const uint32_t lo = min(est >= 127u ? est - 127u : 0u, 65535u - 255u);
-        const us2 base = {(unsigned short)lo, (unsigned short)lo};
-        const us2 c255 = {255, 255};
-#pragma unroll
-        for (int i = 0; i < ND; ++i) {
-            us2 a = __builtin_elementwise_min(__builtin_elementwise_sub_sat(as_us2(r[2 * i]), base), c255);
-            us2 b = __builtin_elementwise_min(__builtin_elementwise_sub_sat(as_us2(r[2 * i + 1]), base), c255);
-            y[i] = __builtin_amdgcn_perm(as_u32(b), as_u32(a), 0x06040200u);
-        }
-        const uint32_t ys = byte_select<G, ND>(y, r0, ntot);
-        if (ys >= 1u && ys <= 254u) {
-            v0 = lo + ys;
-            if (r1 != r0) {
-                cle = byte_count_lt<G, ND>(y, ys + 1, ntot);  // #{x < v0 + 1}
-                have_cle = true;
-            }
-        } else {
-            v0 = select_rank_u16<G, NW>(r, r0, ntot);
-        }
-    } else {
-        v0 = select_rank_u16<G, NW>(r, r0, ntot);
-    }
-    if (r1 == r0) { v1 = v0; return; }
-    if (!have_cle) cle = v0 >= MAXV ? (uint32_t)ntot : count_le<G, NW>(r, v0, ntot);
-    v1 = cle >= r1 + 1 ? v0 : min_greater<G, NW>(r, v0);
+// Neighbour rank r1 = r0 + 1 from v0 and cle = #{x <= v0}.
+template <int G, int NW>
+__device__ __forceinline__ uint32_t neighbour(const uint32_t (&r)[NW], uint32_t v0, uint32_t cle, uint32_t r1) {
+    return cle >= r1 + 1 ? v0 : min_greater<G, NW>(r, v0);
 }
 
-template <int B, int C>
-struct FastPixel {
-    const OutlierK& k;
-    const uint64_t* off;  // LDS offsets
-    uint64_t pix;
-    __device__ __forceinline__ void sample(uint32_t t, uint32_t (&s)[4]) const {
+// Values at ranks (r0, r1), r1 in {r0, r0 + 1}, of every channel.
+//  u8:  one byte level; the neighbour comes from one more byte count.
+//  u16, WINDOW (medians of absolute thresholds): an 8-step byte search inside the window
+//       [lo, lo + 255] around the estimate; a channel's result is exact when it ends strictly
+//       inside (1..254), otherwise that channel runs the exact two-level search.
+//  u16 otherwise: the exact two-level search.
+// Branches depend on group-uniform values only.
+template <int B, int G, int NW, int C, bool WINDOW>
+__device__ __forceinline__ void rank_pairs(const uint32_t (&raw)[C][NW], uint32_t r0, uint32_t r1, int32_t ntot,
+                                           uint32_t (&v0)[C], uint32_t (&v1)[C]) {
+    constexpr int ND = NW / 2;
+    constexpr uint32_t MAXV = B == 1 ? 255u : 65535u;
+    const uint32_t kk[1] = {r0};
+    uint32_t sel[C][1];
+    if constexpr (B == 1) {
+        uint32_t y[C][ND];
 #pragma unroll
-        for (int ch = 0; ch < C; ++ch) s[ch] = load_sample<B>(k.stack, off[t] + pix + (uint64_t)ch);
+        for (int c = 0; c < C; ++c) pack_low_bytes<NW>(raw[c], y[c]);
+        byte_select_multi<G, ND, C, 1>(y, kk, ntot, sel);
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            v0[c] = sel[c][0];
+            if (r1 == r0) { v1[c] = v0[c]; continue; }
+            const uint32_t cle = v0[c] >= 255u ? (uint32_t)ntot : byte_count_lt<G, ND>(y[c], v0[c] + 1, ntot);
+            v1[c] = neighbour<G, NW>(raw[c], v0[c], cle, r1);
+        }
+    } else if constexpr (WINDOW) {
+        uint32_t y[C][ND], lo[C];
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            const uint32_t est = group_estimate<G, NW>(raw[c]);
+            lo[c] = min(est >= 127u ? est - 127u : 0u, 65535u - 255u);
+            pack_window<NW>(raw[c], lo[c], y[c]);
+        }
+        byte_select_multi<G, ND, C, 1>(y, kk, ntot, sel);
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            const uint32_t ys = sel[c][0];
+            uint32_t cle = 0;
+            if (ys >= 1u && ys <= 254u) {
+                v0[c] = lo[c] + ys;
+                if (r1 != r0) cle = byte_count_lt<G, ND>(y[c], ys + 1, ntot);  // #{x < v0 + 1}
+            } else {
+                uint32_t one[1][NW];
+#pragma unroll
+                for (int i = 0; i < NW; ++i) one[0][i] = raw[c][i];
+                uint32_t v[1];
+                select_rank_u16_multi<G, NW, 1>(one, r0, ntot, v);
+                v0[c] = v[0];
+                if (r1 != r0) cle = v0[c] >= MAXV ? (uint32_t)ntot : count_le<G, NW>(raw[c], v0[c], ntot);
+            }
+            v1[c] = r1 == r0 ? v0[c] : neighbour<G, NW>(raw[c], v0[c], cle, r1);
+        }
+    } else {
+        select_rank_u16_multi<G, NW, C>(raw, r0, ntot, v0);
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            if (r1 == r0) { v1[c] = v0[c]; continue; }
+            const uint32_t cle = v0[c] >= MAXV ? (uint32_t)ntot : count_le<G, NW>(raw[c], v0[c], ntot);
+            v1[c] = neighbour<G, NW>(raw[c], v0[c], cle, r1);
+        }
     }
-};
+}
 
 // src/chrono.rs:257-278 on one frame's C samples (f32, reference operation order).
 template <int C>
@@ -251,259 +298,362 @@ __device__ __forceinline__ float frame_dist(const uint32_t (&s)[4], const float 
     return d;
 }
 
-// One tile: P = 256/G pixels of row y starting at pixel bx * P.  s_off holds the element
-// offsets of the N frame positions (LDS).  Lanes that do not finish a pixel return early.
+// Per-tile LDS: the frame-offset table, every lane's classification summary and every
+// pixel's order statistics.
+template <int C, int G, int F>
+struct TileShared {
+    static constexpr int P = 256 / G;
+    static constexpr int MW = F > 32 ? 2 : 1;  // mask words per lane
+    struct Pix {
+        float med[4];
+        float iqr_inv[4];
+        uint32_t s0[4];       // frame 0 samples
+        uint32_t sum_all[4];  // average modes
+        uint32_t sum_out[4];
+    };
+    uint64_t off[G * F];       // frame byte offsets; positions >= N repeat position N-1
+    uint32_t mask[256 * MW];   // lane l, word w: outlier bits of frames t0 + 32w + b
+    float lmax[256];           // lane's largest outlier distance (valid when lmaxi != ~0)
+    uint32_t lmaxi[256];       // its first frame
+    Pix px[P];
+};
+
+// Fills the frame-offset table of a tile (all 256 lanes take part): byte offsets of the
+// frame positions, the last one repeated up to G*F entries.
+template <int B>
+__device__ __forceinline__ void load_offsets(uint64_t* off, uint32_t count, const uint64_t* __restrict__ pos_off,
+                                             uint32_t n) {
+    for (uint32_t i = threadIdx.x; i < count; i += 256) off[i] = pos_off[min(i, n - 1)] * B;
+}
+
+// Loads frames [t0, t0 + F) of one pixel into raw[c][j/2] (frame j in half j&1); base is
+// the pixel's address in stack slot 0 and off holds the frame byte offsets.
+template <int B, int C, int F>
+__device__ __forceinline__ void load_block(const uint8_t* __restrict__ base, const uint64_t* off, uint32_t t0,
+                                           uint32_t (&raw)[C][F / 2]) {
+#pragma unroll
+    for (int i = 0; i < F / 2; ++i) {
+        const uint8_t* p0 = base + off[t0 + 2 * i];
+        const uint8_t* p1 = base + off[t0 + 2 * i + 1];
+        if constexpr (B == 2 && C == 3) {
+            uint32_t d0, d1;  // channels 0-1 of each frame (2-byte aligned dword)
+            __builtin_memcpy(&d0, __builtin_assume_aligned(p0, 2), 4);
+            __builtin_memcpy(&d1, __builtin_assume_aligned(p1, 2), 4);
+            raw[0][i] = __builtin_amdgcn_perm(d1, d0, 0x05040100u);
+            raw[1][i] = __builtin_amdgcn_perm(d1, d0, 0x07060302u);
+            uint32_t e0, e1;  // channels 1-2 of each frame
+            __builtin_memcpy(&e0, __builtin_assume_aligned(p0 + 2, 2), 4);
+            __builtin_memcpy(&e1, __builtin_assume_aligned(p1 + 2, 2), 4);
+            raw[2][i] = __builtin_amdgcn_perm(e1, e0, 0x07060302u);
+        } else if constexpr (B == 2 && C == 4) {
+            const uint2 q0 = *reinterpret_cast<const uint2*>(p0);
+            const uint2 q1 = *reinterpret_cast<const uint2*>(p1);
+            raw[0][i] = __builtin_amdgcn_perm(q1.x, q0.x, 0x05040100u);
+            raw[1][i] = __builtin_amdgcn_perm(q1.x, q0.x, 0x07060302u);
+            raw[2][i] = __builtin_amdgcn_perm(q1.y, q0.y, 0x05040100u);
+            raw[3][i] = __builtin_amdgcn_perm(q1.y, q0.y, 0x07060302u);
+        } else if constexpr (B == 1 && C == 4) {
+            const uint32_t d0 = *reinterpret_cast<const uint32_t*>(p0);
+            const uint32_t d1 = *reinterpret_cast<const uint32_t*>(p1);
+#pragma unroll
+            for (int c = 0; c < 4; ++c)  // byte c of each frame into the low byte of each half
+                raw[c][i] = __builtin_amdgcn_perm(d1, d0, 0x0C000C00u | ((uint32_t)(4 + c) << 16) | (uint32_t)c);
+        } else if constexpr (B == 1 && C == 3) {
+            uint16_t w0, w1;  // channels 0-1 of each frame
+            __builtin_memcpy(&w0, p0, 2);
+            __builtin_memcpy(&w1, p1, 2);
+            raw[0][i] = __builtin_amdgcn_perm((uint32_t)w1, (uint32_t)w0, 0x0C040C00u);
+            raw[1][i] = __builtin_amdgcn_perm((uint32_t)w1, (uint32_t)w0, 0x0C050C01u);
+            us2 v;
+            v.x = p0[2];
+            v.y = p1[2];
+            raw[2][i] = as_u32(v);
+        } else {
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                us2 v;
+                v.x = (unsigned short)load_sample<B>(p0, c);
+                v.y = (unsigned short)load_sample<B>(p1, c);
+                raw[c][i] = as_u32(v);
+            }
+        }
+    }
+}
+
+// One tile: P = 256/G pixels of row y starting at pixel bx * P.  sh.off must hold the
+// tile's frame offsets (load_offsets) and be visible to all lanes.  Every lane of the
+// workgroup must call this (it synchronises once).
 template <int B, int C, int G, int F, bool ABS>
 __device__ __forceinline__ void outlier_tile(const OutlierK& k, uint32_t bx, uint32_t y,
-                                             const uint64_t* __restrict__ s_off) {
+                                             TileShared<C, G, F>& sh) {
     static_assert(F % 4 == 0 && F <= 64 && G <= 8, "tile shape");
-    constexpr int P = 256 / G;
+    using TS = TileShared<C, G, F>;
+    constexpr int P = TS::P;
+    constexpr int MW = TS::MW;
     constexpr int NW = F / 2;
     constexpr uint32_t MAXV = B == 1 ? 255u : 65535u;
     constexpr uint32_t NONE = 0xFFFFFFFFu;
     const uint32_t N = k.n;
 
-    const uint32_t g = threadIdx.x % G;
-    const uint32_t p = threadIdx.x / G;
-    const uint32_t x = bx * P + p;
-    const bool px_ok = x < k.out_w;
-    const uint64_t pix = ((uint64_t)y * k.stack_w + (px_ok ? x : 0u)) * C;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t g = lane % G;
     const uint32_t t0 = g * F;
     const int32_t ntot = G * F;
-
-    // ---- load the lane's frame block.  Loads are unconditional (frame index clamped to
-    // N-1) so they issue back to back; pads above N are then set to MAXV.
-    uint32_t raw[C][NW];
     {
-        us2 rv[C][NW];
-#pragma unroll
-        for (int j = 0; j < F; ++j) {
-            const uint32_t t = min(t0 + j, N - 1);
-            const uint64_t base = s_off[t] + pix;
-#pragma unroll
-            for (int c = 0; c < C; ++c) {
-                const unsigned short v = (unsigned short)load_sample<B>(k.stack, base + (uint64_t)c);
-                if (j & 1) rv[c][j >> 1].y = v;
-                else rv[c][j >> 1].x = v;
-            }
-        }
-#pragma unroll
-        for (int c = 0; c < C; ++c)
-#pragma unroll
-            for (int i = 0; i < NW; ++i) raw[c][i] = as_u32(rv[c][i]);
-    }
-    if (N < (uint32_t)(G * F)) {  // grid-uniform: only stacks that do not fill the tile
-#pragma unroll
-        for (int i = 0; i < NW; ++i) {
-            const uint32_t t = t0 + 2 * i;
-#pragma unroll
-            for (int c = 0; c < C; ++c) {
-                uint32_t r = raw[c][i];
-                r = t + 1 >= N ? (r & 0xFFFFu) | (MAXV << 16) : r;
-                r = t >= N ? MAXV | (MAXV << 16) : r;
-                raw[c][i] = r;
-            }
-        }
-    }
+        const uint32_t p = lane / G;
+        const uint32_t x = bx * P + p;
+        const bool px_ok = x < k.out_w;
+        const uint64_t pix = ((uint64_t)y * k.stack_w + (px_ok ? x : 0u)) * C;
 
-    // ---- medians / inverse IQR per weighted channel
-    const RankPlan plan = make_rank_plan(N, ABS);
-    float med[4] = {0, 0, 0, 0}, iqr_inv[4] = {0, 0, 0, 0};
-    // One code path serves every channel (a rolled loop keeps the kernel small); the
-    // channel's registers are selected into `cur` with static register indices.
-#pragma unroll 1
-    for (int c = 0; c < C; ++c) {
-        if (k.w[c] == 0.0f) continue;
-        uint32_t cur[NW];
-#pragma unroll
-        for (int i = 0; i < NW; ++i) cur[i] = raw[0][i];
-#pragma unroll
-        for (int cc = 1; cc < C; ++cc)
-#pragma unroll
-            for (int i = 0; i < NW; ++i) cur[i] = c == cc ? raw[cc][i] : cur[i];
-        uint32_t vals[6] = {0, 0, 0, 0, 0, 0};
-        uint32_t m0, m1;
-        rank_pair<B, G, NW, ABS>(cur, plan.m0, plan.m1, ntot, m0, m1);
-        uint32_t a0 = 0, a1 = 0, b0 = 0, b1 = 0;
-        if constexpr (!ABS) {
-            rank_pair<B, G, NW, false>(cur, plan.a0, plan.a1, ntot, a0, a1);
-            rank_pair<B, G, NW, false>(cur, plan.b0, plan.b1, ntot, b0, b1);
-        }
-#pragma unroll
-        for (int i = 0; i < 6; ++i) {
-            const uint32_t rk = plan.rank[i];
-            uint32_t v = 0;
-            if (rk == plan.m0) v = m0;
-            else if (rk == plan.m1) v = m1;
-            else if (rk == plan.a0) v = a0;
-            else if (rk == plan.a1) v = a1;
-            else if (rk == plan.b0) v = b0;
-            else if (rk == plan.b1) v = b1;
-            vals[i] = v;
-        }
-        float mc, ic;
-        stats_from_ranks(plan, vals, ABS, &mc, &ic);
-#pragma unroll
-        for (int cc = 0; cc < C; ++cc) {
-            med[cc] = c == cc ? mc : med[cc];
-            iqr_inv[cc] = c == cc ? ic : iqr_inv[cc];
-        }
-    }
-
-    // ---- classification over the lane's frames
-    float wi[4], sg[4];
-    bool unit = ABS, pos = true;
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-        wi[c] = ABS ? k.w[c] : k.w[c] * iqr_inv[c];
-        sg[c] = signum_f(k.w[c]);
-        if (c < C) {
-            unit = unit && k.w[c] == 1.0f;
-            pos = pos && !(sg[c] < 0.0f);
-        }
-    }
-    const float tsq = k.th.min * k.th.min;
-    const bool want_max = k.outlier == 2 || k.stats != nullptr;
-    const bool want_sums = k.background == 2 || k.outlier == 3;
-    const bool full_tile = N == (uint32_t)(G * F);
-    uint32_t m_lo = 0, m_hi = 0;
-    float lmax = 0.0f;
-    uint32_t lmaxi = NONE;
-    uint32_t sum_all[4] = {0, 0, 0, 0}, sum_out[4] = {0, 0, 0, 0};
-
-    // Bound check: with non-negative weights every frame's f32 distance is at most the
-    // same f32 expression on the per-channel maximum deviation from the median (correct
-    // rounding is monotone), so when that bound is below the threshold no frame of this
-    // lane can be an outlier and the per-frame pass is skipped.
-    bool may_outlier = true;
-    if (pos && full_tile) {
-        float bound = 0.0f;
-#pragma unroll
-        for (int c = 0; c < C; ++c) {
-            us2 mn = {0xFFFF, 0xFFFF}, mx = {0, 0};
+        // ---- the lane's frame block; pads above N are set to MAXV
+        uint32_t raw[C][NW];
+        load_block<B, C, F>(k.stack + pix * B, sh.off, t0, raw);
+        if (N < (uint32_t)(G * F)) {  // grid-uniform: only stacks that do not fill the tile
 #pragma unroll
             for (int i = 0; i < NW; ++i) {
-                mn = __builtin_elementwise_min(mn, as_us2(raw[c][i]));
-                mx = __builtin_elementwise_max(mx, as_us2(raw[c][i]));
-            }
-            const float lo = (float)min((uint32_t)mn.x, (uint32_t)mn.y);
-            const float hi = (float)max((uint32_t)mx.x, (uint32_t)mx.y);
-            const float dev = fmaxf(med[c] - lo, hi - med[c]);
-            const float a = unit ? dev : wi[c] * dev;
-            bound = c == 0 ? a * a : bound + a * a;
-        }
-        may_outlier = bound >= tsq;
-    }
-
-    auto classify = [&](auto unit_t, auto pos_t, auto max_t, auto sums_t) {
-        constexpr bool UNIT = decltype(unit_t)::value, POS = decltype(pos_t)::value;
-        constexpr bool MAXT = decltype(max_t)::value, SUMS = decltype(sums_t)::value;
+                const uint32_t t = t0 + 2 * i;
 #pragma unroll
-        for (int j = 0; j < F; ++j) {
-            uint32_t xs[4];
-            float d = 0.0f;
+                for (int c = 0; c < C; ++c) {
+                    uint32_t r = raw[c][i];
+                    r = t + 1 >= N ? (r & 0xFFFFu) | (MAXV << 16) : r;
+                    r = t >= N ? MAXV | (MAXV << 16) : r;
+                    raw[c][i] = r;
+                }
+            }
+        }
+
+        // ---- medians / inverse IQR per channel
+        const RankPlan plan = make_rank_plan(N, ABS);
+        float med[4] = {0, 0, 0, 0}, iqr_inv[4] = {0, 0, 0, 0};
+        {
+            uint32_t m0[C], m1[C];
+            rank_pairs<B, G, NW, C, ABS>(raw, plan.m0, plan.m1, ntot, m0, m1);
+            uint32_t a0[C], a1[C], b0[C], b1[C];
+            if constexpr (!ABS) {
+                rank_pairs<B, G, NW, C, false>(raw, plan.a0, plan.a1, ntot, a0, a1);
+                rank_pairs<B, G, NW, C, false>(raw, plan.b0, plan.b1, ntot, b0, b1);
+            }
 #pragma unroll
             for (int c = 0; c < C; ++c) {
-                const uint32_t w32 = raw[c][j >> 1];
-                xs[c] = (j & 1) ? (w32 >> 16) : (w32 & 0xFFFFu);
-                const float diff = med[c] - (float)xs[c];
-                const float a = UNIT ? diff : wi[c] * diff;
-                float term = a * a;
-                if constexpr (!POS) term = sg[c] * term;
-                d = c == 0 ? term : d + term;  // 0 + term0 == term0 up to the sign of zero
+                uint32_t vals[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+                for (int i = 0; i < 6; ++i) {  // equal ranks carry equal values
+                    const uint32_t rk = plan.rank[i];
+                    uint32_t v = 0;
+                    if constexpr (!ABS) {
+                        if (rk == plan.b1) v = b1[c];
+                        if (rk == plan.b0) v = b0[c];
+                        if (rk == plan.a1) v = a1[c];
+                        if (rk == plan.a0) v = a0[c];
+                    }
+                    if (rk == plan.m1) v = m1[c];
+                    if (rk == plan.m0) v = m0[c];
+                    vals[i] = v;
+                }
+                float mc, ic;
+                stats_from_ranks(plan, vals, ABS, &mc, &ic);
+                // channels without weight keep zero statistics
+                med[c] = k.w[c] == 0.0f ? 0.0f : mc;
+                iqr_inv[c] = k.w[c] == 0.0f ? 0.0f : ic;
             }
-            const uint32_t o = d >= tsq ? 1u : 0u;  // pads are masked out after the loop
-            if (j < 32) m_lo |= o << j;
-            else m_hi |= o << (j - 32);
-            if constexpr (MAXT || SUMS) {
-                const bool ov = o && (t0 + j < N);
+        }
+
+        // ---- classification over the lane's frames
+        float wi[4], sg[4];
+        bool unit = ABS, pos = true;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            wi[c] = ABS ? k.w[c] : k.w[c] * iqr_inv[c];
+            sg[c] = signum_f(k.w[c]);
+            if (c < C) {
+                unit = unit && k.w[c] == 1.0f;
+                pos = pos && !(sg[c] < 0.0f);
+            }
+        }
+        const float tsq = k.th.min * k.th.min;
+        const bool want_max = k.outlier == 2 || k.stats != nullptr;
+        const bool want_sums = k.background == 2 || k.outlier == 3;
+        const bool full_tile = N == (uint32_t)(G * F);
+        uint32_t m[MW];
+#pragma unroll
+        for (int w = 0; w < MW; ++w) m[w] = 0;
+        float lmax = tsq;
+        uint32_t lmaxi = NONE;
+        uint32_t sum_all[4] = {0, 0, 0, 0}, sum_out[4] = {0, 0, 0, 0};
+
+        // Bound check: with non-negative weights every frame's f32 distance is at most the
+        // same f32 expression on the per-channel maximum deviation from the median (correct
+        // rounding is monotone), so when that bound is below the threshold no frame of this
+        // lane can be an outlier and the per-frame pass is skipped.
+        bool may_outlier = true;
+        if (pos && full_tile) {
+            float bound = 0.0f;
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                us2 mn = {0xFFFF, 0xFFFF}, mx = {0, 0};
+#pragma unroll
+                for (int i = 0; i < NW; ++i) {
+                    mn = __builtin_elementwise_min(mn, as_us2(raw[c][i]));
+                    mx = __builtin_elementwise_max(mx, as_us2(raw[c][i]));
+                }
+                const float lo = (float)min((uint32_t)mn.x, (uint32_t)mn.y);
+                const float hi = (float)max((uint32_t)mx.x, (uint32_t)mx.y);
+                const float dev = fmaxf(med[c] - lo, hi - med[c]);
+                const float a = unit ? dev : wi[c] * dev;
+                bound = c == 0 ? a * a : bound + a * a;
+            }
+            may_outlier = bound >= tsq;
+        }
+
+        // Frames run from last to first so that the mask shifts in bit j for frame j; the
+        // largest distance keeps the lowest frame among ties (>=).  lmax starts at the
+        // threshold, so `d >= lmax` also implies that the frame is an outlier.
+        auto classify = [&](auto unit_t, auto pos_t, auto max_t, auto sums_t, auto check_t) {
+            constexpr bool UNIT = decltype(unit_t)::value, POS = decltype(pos_t)::value;
+            constexpr bool MAXT = decltype(max_t)::value, SUMS = decltype(sums_t)::value;
+            constexpr bool CHECK = decltype(check_t)::value;
+#pragma unroll
+            for (int j = F - 1; j >= 0; --j) {
+                uint32_t xs[4];
+                float d = 0.0f;
+#pragma unroll
+                for (int c = 0; c < C; ++c) {
+                    const uint32_t w32 = raw[c][j >> 1];
+                    xs[c] = (j & 1) ? (w32 >> 16) : (w32 & 0xFFFFu);
+                    const float diff = med[c] - (float)xs[c];
+                    const float a = UNIT ? diff : wi[c] * diff;
+                    float term = a * a;
+                    if constexpr (!POS) term = sg[c] * term;
+                    d = c == 0 ? term : d + term;  // 0 + term0 == term0 up to the sign of zero
+                }
+                const bool o = d >= tsq;  // pads are masked out after the loop
+                m[j >> 5] = (m[j >> 5] << 1) | (o ? 1u : 0u);
+                bool in = true;
+                if constexpr (CHECK) in = t0 + (uint32_t)j < N;
                 if constexpr (MAXT) {
-                    if (ov && d > lmax) { lmax = d; lmaxi = t0 + j; }
+                    if (in && d >= lmax) { lmax = d; lmaxi = t0 + (uint32_t)j; }
                 }
                 if constexpr (SUMS) {
 #pragma unroll
-                    for (int c = 0; c < C; ++c) sum_out[c] += ov ? xs[c] : 0u;
+                    for (int c = 0; c < C; ++c) sum_out[c] += (o && in) ? xs[c] : 0u;
                 }
             }
+        };
+        using T_ = std::true_type;
+        using F_ = std::false_type;
+        if (may_outlier) {
+            if (!full_tile) classify(F_{}, F_{}, T_{}, T_{}, T_{});
+            else if (want_max) {
+                if (want_sums) classify(F_{}, F_{}, T_{}, T_{}, F_{});
+                else if (unit) classify(T_{}, T_{}, T_{}, F_{}, F_{});
+                else if (pos) classify(F_{}, T_{}, T_{}, F_{}, F_{});
+                else classify(F_{}, F_{}, T_{}, F_{}, F_{});
+            } else {
+                if (want_sums) classify(F_{}, F_{}, F_{}, T_{}, F_{});
+                else if (unit) classify(T_{}, T_{}, F_{}, F_{}, F_{});
+                else if (pos) classify(F_{}, T_{}, F_{}, F_{}, F_{});
+                else classify(F_{}, F_{}, F_{}, F_{}, F_{});
+            }
         }
-    };
-    using T_ = std::true_type;
-    using F_ = std::false_type;
-    if (may_outlier) {
-        if (want_max) {
-            if (want_sums) classify(F_{}, F_{}, T_{}, T_{});
-            else if (unit) classify(T_{}, T_{}, T_{}, F_{});
-            else if (pos) classify(F_{}, T_{}, T_{}, F_{});
-            else classify(F_{}, F_{}, T_{}, F_{});
-        } else {
-            if (want_sums) classify(F_{}, F_{}, F_{}, T_{});
-            else if (unit) classify(T_{}, T_{}, F_{}, F_{});
-            else if (pos) classify(F_{}, T_{}, F_{}, F_{});
-            else classify(F_{}, F_{}, F_{}, F_{});
-        }
-    }
-    if (want_sums) {
-        // all-frame sums through v_sad_u16 against zero; pads (MAXV) are taken back out
-        const uint32_t npad = t0 + F > N ? min(t0 + F - N, (uint32_t)F) : 0u;
+        if (want_sums) {
+            // all-frame sums through v_sad_u16 against zero; pads (MAXV) are taken back out
+            const uint32_t npad = t0 + F > N ? min(t0 + F - N, (uint32_t)F) : 0u;
 #pragma unroll
-        for (int c = 0; c < C; ++c) {
-            uint32_t acc = 0;
+            for (int c = 0; c < C; ++c) {
+                uint32_t acc = 0;
 #pragma unroll
-            for (int i = 0; i < NW; ++i) acc = __builtin_amdgcn_sad_u16(raw[c][i], 0u, acc);
-            sum_all[c] = acc - npad * MAXV;
+                for (int i = 0; i < NW; ++i) acc = __builtin_amdgcn_sad_u16(raw[c][i], 0u, acc);
+                sum_all[c] = (uint32_t)grp_sum<G>((int32_t)(acc - npad * MAXV));
+                sum_out[c] = (uint32_t)grp_sum<G>((int32_t)sum_out[c]);
+            }
         }
-    }
 
-    // ---- drop pads from the masks, then group reductions
-    uint32_t v_lo, v_hi;
-    {
-        const int32_t nv = min((int32_t)N - (int32_t)t0, (int32_t)F);  // valid frames in this block
-        v_lo = nv >= 32 ? 0xFFFFFFFFu : (nv <= 0 ? 0u : (1u << nv) - 1u);
-        v_hi = nv >= 64 ? 0xFFFFFFFFu : (nv <= 32 ? 0u : (1u << (nv - 32)) - 1u);
-    }
-    m_lo &= v_lo;
-    m_hi &= v_hi;
-    const uint32_t num = (uint32_t)grp_sum<G>((int32_t)(__builtin_popcount(m_lo) + __builtin_popcount(m_hi)));
-    uint32_t max_index = 0;
-    float max_d = 0.0f;
-    if (want_max) {
-        max_d = __builtin_bit_cast(float, grp_max_u32<G>(__builtin_bit_cast(uint32_t, lmax)));  // d >= 0
-        const uint32_t cand = (lmaxi != NONE && lmax == max_d && max_d > 0.0f) ? lmaxi : NONE;
-        max_index = grp_min_u32<G>(cand);
-        if (max_index == NONE) max_index = 0;
-    }
-    if (want_sums) {
+        // ---- leave the lane's summary and the pixel's statistics in LDS
+        {
+            const int32_t nv = min((int32_t)N - (int32_t)t0, (int32_t)F);  // valid frames in this block
 #pragma unroll
-        for (int c = 0; c < C; ++c) {
-            sum_all[c] = (uint32_t)grp_sum<G>((int32_t)sum_all[c]);
-            sum_out[c] = (uint32_t)grp_sum<G>((int32_t)sum_out[c]);
+            for (int w = 0; w < MW; ++w) {
+                const int32_t nw = nv - 32 * w;
+                const uint32_t vm = nw >= 32 ? 0xFFFFFFFFu : (nw <= 0 ? 0u : (1u << nw) - 1u);
+                sh.mask[lane * MW + w] = m[w] & vm;
+            }
+            sh.lmax[lane] = lmax;
+            sh.lmaxi[lane] = lmaxi;
+        }
+        if (g == 0) {
+            typename TS::Pix& rec = sh.px[p];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                rec.med[c] = med[c];
+                rec.iqr_inv[c] = iqr_inv[c];
+            }
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                rec.s0[c] = raw[c][0] & 0xFFFFu;
+                rec.sum_all[c] = sum_all[c];
+                rec.sum_out[c] = sum_out[c];
+            }
         }
     }
-    const uint32_t lane_first = m_lo ? t0 + __builtin_ctz(m_lo) : (m_hi ? t0 + 32 + __builtin_ctz(m_hi) : NONE);
-    const uint32_t lane_last = m_hi ? t0 + 63 - __builtin_clz(m_hi) : (m_lo ? t0 + 31 - __builtin_clz(m_lo) : 0u);
-    const uint32_t first_idx = grp_min_u32<G>(lane_first);
-    const uint32_t last_idx = grp_max_u32<G>(lane_last);
-    // first valid non-outlier frame (background "first")
-    const uint32_t i_lo = v_lo & ~m_lo, i_hi = v_hi & ~m_hi;
-    const uint32_t lane_non = i_lo ? t0 + __builtin_ctz(i_lo) : (i_hi ? t0 + 32 + __builtin_ctz(i_hi) : NONE);
-    const uint32_t first_non = grp_min_u32<G>(lane_non);
-    // every lane of the group learns the group's masks (for in-order outlier walks)
-    const uint32_t lane = __lane_id();
-    const uint32_t gbase = lane & ~(uint32_t)(G - 1);
-    uint32_t gm[G][2];
+    __syncthreads();
+    if (lane >= (uint32_t)P) return;
+
+    // ---- finish one pixel per lane: background + outlier selection (src/chrono.rs:290-493)
+    const uint32_t p = lane;
+    const uint32_t x = bx * P + p;
+    if (x >= k.out_w) return;
+    const uint64_t pix = ((uint64_t)y * k.stack_w + x) * C;
+    const typename TS::Pix& rec = sh.px[p];
+    float med[4], iqr_inv[4], wi[4], sg[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        med[c] = rec.med[c];
+        iqr_inv[c] = rec.iqr_inv[c];
+        wi[c] = ABS ? k.w[c] : k.w[c] * iqr_inv[c];
+        sg[c] = signum_f(k.w[c]);
+    }
+    uint32_t gm[G][MW];
+#pragma unroll
+    for (int q = 0; q < G; ++q)
+#pragma unroll
+        for (int w = 0; w < MW; ++w) gm[q][w] = sh.mask[(p * G + q) * MW + w];
+
+    uint32_t num = 0, first_idx = NONE, last_idx = 0, first_non = NONE;
 #pragma unroll
     for (int q = 0; q < G; ++q) {
-        gm[q][0] = (uint32_t)__shfl((int)m_lo, (int)(gbase + q));
-        gm[q][1] = (uint32_t)__shfl((int)m_hi, (int)(gbase + q));
+#pragma unroll
+        for (int w = 0; w < MW; ++w) {
+            const uint32_t mm = gm[q][w];
+            const uint32_t base = (uint32_t)q * F + 32u * w;
+            num += __builtin_popcount(mm);
+            if (first_idx == NONE && mm) first_idx = base + __builtin_ctz(mm);
+            if (mm) last_idx = base + 31 - __builtin_clz(mm);
+            // valid non-outlier frames of this word
+            const int32_t nw = min((int32_t)N - (int32_t)base, 32);
+            const uint32_t vm = nw >= 32 ? 0xFFFFFFFFu : (nw <= 0 ? 0u : (1u << nw) - 1u);
+            const uint32_t non = vm & ~mm;
+            if (first_non == NONE && non) first_non = base + __builtin_ctz(non);
+        }
     }
+    float max_d = 0.0f;
+    uint32_t max_index = 0;
+    if (k.outlier == 2 || k.stats != nullptr) {
+#pragma unroll
+        for (int q = 0; q < G; ++q) {
+            const uint32_t li = sh.lmaxi[p * G + q];
+            const float lm = sh.lmax[p * G + q];
+            if (li != NONE && lm > max_d) { max_d = lm; max_index = li; }
+        }
+    }
+    if (first_idx == NONE) first_idx = 0;
 
-    if (g != 0 || !px_ok) return;
-
-    // ---- finish one pixel: background + outlier selection (src/chrono.rs:290-493)
     constexpr uint32_t maxv = MAXV;
-    FastPixel<B, C> P_{k, s_off, pix};
+    auto sample = [&](uint32_t t, uint32_t (&s)[4]) {
+#pragma unroll
+        for (int ch = 0; ch < C; ++ch) s[ch] = load_sample<B>(k.stack + sh.off[t], pix + (uint64_t)ch);
+    };
     auto dist_of = [&](uint32_t t, uint32_t (&s)[4]) -> float {
-        P_.sample(t, s);
+        sample(t, s);
         return frame_dist<C>(s, med, wi, sg);
     };
     const uint64_t pix_id = (uint64_t)y * k.out_w + x;
@@ -514,16 +664,16 @@ __device__ __forceinline__ void outlier_tile(const OutlierK& k, uint32_t bx, uin
     if (k.background == 2) {
         float mean[4];
 #pragma unroll
-        for (int c = 0; c < C; ++c) mean[c] = (float)sum_all[c] / (float)N;
+        for (int c = 0; c < C; ++c) mean[c] = (float)rec.sum_all[c] / (float)N;
         if (num == 1) {
-            P_.sample(first_idx, s);
+            sample(first_idx, s);
 #pragma unroll
             for (int c = 0; c < C; ++c)
                 pixel[c] = sat_cast(roundf(mean[c] * ((float)N / (float)(N - 1)) - (float)s[c] / (float)N), maxv);
         } else if (num > 1) {
 #pragma unroll
             for (int c = 0; c < C; ++c)
-                pixel[c] = sat_cast(roundf(mean[c] * ((float)N / (float)(N - num)) - (float)sum_out[c] / (float)N), maxv);
+                pixel[c] = sat_cast(roundf(mean[c] * ((float)N / (float)(N - num)) - (float)rec.sum_out[c] / (float)N), maxv);
         } else {
 #pragma unroll
             for (int c = 0; c < C; ++c) pixel[c] = sat_cast(roundf(mean[c]), maxv);
@@ -545,11 +695,11 @@ __device__ __forceinline__ void outlier_tile(const OutlierK& k, uint32_t bx, uin
 #pragma unroll
                 for (int q = G - 1; q >= 0; --q) {
 #pragma unroll
-                    for (int w = 1; w >= 0; --w) {
-                        uint32_t m = gm[q][w];
-                        while (m) {
-                            const uint32_t b = 31 - __builtin_clz(m);
-                            m &= ~(1u << b);
+                    for (int w = MW - 1; w >= 0; --w) {
+                        uint32_t mm = gm[q][w];
+                        while (mm) {
+                            const uint32_t b = 31 - __builtin_clz(mm);
+                            mm &= ~(1u << b);
                             const uint32_t a = (uint32_t)q * F + (uint32_t)w * 32 + b;
                             --j;
                             const uint32_t bb = N - 1 - j;
@@ -561,11 +711,11 @@ __device__ __forceinline__ void outlier_tile(const OutlierK& k, uint32_t bx, uin
                 bg_index = pos;
             }
         }
-        if (bg_index == 0) {  // frame 0 is this lane's first register half
+        if (bg_index == 0) {
 #pragma unroll
-            for (int c = 0; c < C; ++c) s[c] = raw[c][0] & 0xFFFFu;
+            for (int c = 0; c < C; ++c) s[c] = rec.s0[c];
         } else {
-            P_.sample(bg_index, s);
+            sample(bg_index, s);
         }
 #pragma unroll
         for (int c = 0; c < C; ++c) pixel[c] = s[c];
@@ -589,12 +739,12 @@ __device__ __forceinline__ void outlier_tile(const OutlierK& k, uint32_t bx, uin
             for (int qq = 0; qq < G; ++qq) {
                 const int q = fwd ? qq : G - 1 - qq;
 #pragma unroll
-                for (int ww = 0; ww < 2; ++ww) {
-                    const int w = fwd ? ww : 1 - ww;
-                    uint32_t m = gm[q][w];
-                    while (m) {
-                        const uint32_t b = fwd ? (uint32_t)__builtin_ctz(m) : 31 - __builtin_clz(m);
-                        m &= ~(1u << b);
+                for (int ww = 0; ww < MW; ++ww) {
+                    const int w = fwd ? ww : MW - 1 - ww;
+                    uint32_t mm = gm[q][w];
+                    while (mm) {
+                        const uint32_t b = fwd ? (uint32_t)__builtin_ctz(mm) : 31 - __builtin_clz(mm);
+                        mm &= ~(1u << b);
                         const uint32_t t = (uint32_t)q * F + (uint32_t)w * 32 + b;
                         const float d = dist_of(t, s);
                         const float fade = fade_pick(k.fade, (int32_t)t, (int32_t)N, k.frame_offset);
@@ -615,11 +765,11 @@ __device__ __forceinline__ void outlier_tile(const OutlierK& k, uint32_t bx, uin
 #pragma unroll
                 for (int q = 0; q < G; ++q) {
 #pragma unroll
-                    for (int w = 0; w < 2; ++w) {
-                        uint32_t m = gm[q][w];
-                        while (m) {
-                            const uint32_t b = __builtin_ctz(m);
-                            m &= m - 1;
+                    for (int w = 0; w < MW; ++w) {
+                        uint32_t mm = gm[q][w];
+                        while (mm) {
+                            const uint32_t b = __builtin_ctz(mm);
+                            mm &= mm - 1;
                             const uint32_t t = (uint32_t)q * F + (uint32_t)w * 32 + b;
                             uint32_t tmp[4];
                             mean_dist += sqrtf(dist_of(t, tmp));
@@ -627,7 +777,7 @@ __device__ __forceinline__ void outlier_tile(const OutlierK& k, uint32_t bx, uin
                     }
                 }
 #pragma unroll
-                for (int c = 0; c < C; ++c) s[c] = sat_cast(roundf((float)sum_out[c] / (float)num), maxv);
+                for (int c = 0; c < C; ++c) s[c] = sat_cast(roundf((float)rec.sum_out[c] / (float)num), maxv);
                 dist = mean_dist / (float)num;
             } else {
                 sel = k.outlier == 0 ? first_idx : (k.outlier == 1 ? last_idx : max_index);
@@ -661,12 +811,21 @@ __device__ __forceinline__ void outlier_tile(const OutlierK& k, uint32_t bx, uin
     }
 }
 
+// Workgroup b of nb goes to XCD b % 8 in dispatch order; renumbering the tiles gives each
+// XCD one contiguous run, so neighbouring tiles (which share cache lines at their edges)
+// meet in the same L2.
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t nb) {
+    const uint32_t q = nb / 8u, r = nb % 8u, xcd = b % 8u, idx = b / 8u;
+    return xcd * q + min(xcd, r) + idx;
+}
+
 template <int B, int C, int G, int F, bool ABS>
-__global__ void __launch_bounds__(256) outlier_fast_kernel(OutlierK k) {
-    __shared__ uint64_t s_off[G * F];
-    for (uint32_t i = threadIdx.x; i < k.n; i += blockDim.x) s_off[i] = k.pos_off[i];
+__global__ void __launch_bounds__(256) outlier_fast_kernel(OutlierK k, uint32_t tiles_x) {
+    __shared__ TileShared<C, G, F> sh;
+    load_offsets<B>(sh.off, G * F, k.pos_off, k.n);
     __syncthreads();
-    outlier_tile<B, C, G, F, ABS>(k, blockIdx.x, blockIdx.y, s_off);
+    const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
+    outlier_tile<B, C, G, F, ABS>(k, tile % tiles_x, tile / tiles_x, sh);
 }
 
 }  // namespace cp

@@ -32,9 +32,10 @@ inline int bucket_for(uint32_t n) {
 template <int B, int C, int G, int F>
 inline void launch_fast_shape(const OutlierK& k, hipStream_t s) {
     constexpr int P = 256 / G;
-    dim3 grid((k.out_w + P - 1) / P, k.out_h), block(256);
-    if (k.th.absolute) hipLaunchKernelGGL((outlier_fast_kernel<B, C, G, F, true>), grid, block, 0, s, k);
-    else hipLaunchKernelGGL((outlier_fast_kernel<B, C, G, F, false>), grid, block, 0, s, k);
+    const uint32_t tiles_x = (k.out_w + P - 1) / P;
+    dim3 grid(tiles_x * k.out_h), block(256);
+    if (k.th.absolute) hipLaunchKernelGGL((outlier_fast_kernel<B, C, G, F, true>), grid, block, 0, s, k, tiles_x);
+    else hipLaunchKernelGGL((outlier_fast_kernel<B, C, G, F, false>), grid, block, 0, s, k, tiles_x);
 }
 
 template <int B, int C>
