@@ -23,6 +23,8 @@ struct DirectJob {
 
 struct Group {
     uint32_t bps = 0, channels = 0;
+    int shape = 0;
+    bool absolute = false;
     std::vector<BatchJobK> jobs;      // host copies (device pointers patched at upload)
     std::vector<std::vector<uint64_t>> pos;
     std::vector<std::vector<float>> fades;
@@ -41,8 +43,9 @@ bool batch_fast_ok(const cp_stack* st, const OutlierPrep& p) {
 }
 
 int launch_group(const Group& g, int cus, hipStream_t s) {
-    if (g.channels == 3) return launch_batch_b1_c3(g.d_jobs, (uint32_t)g.jobs.size(), g.total, g.d_counter, cus, s);
-    return launch_batch_b1_c4(g.d_jobs, (uint32_t)g.jobs.size(), g.total, g.d_counter, cus, s);
+    const uint32_t nj = (uint32_t)g.jobs.size();
+    if (g.channels == 3) return launch_batch_b1_c3(g.d_jobs, nj, g.total, g.d_counter, g.shape, g.absolute, cus, s);
+    return launch_batch_b1_c4(g.d_jobs, nj, g.total, g.d_counter, g.shape, g.absolute, cus, s);
 }
 
 }  // namespace
@@ -97,13 +100,18 @@ extern "C" CP_API int cp_batch_create(cp_ctx* ctx, const cp_job* jobs, uint32_t 
             if (shape > kBatchShapeMax) shape = kBatchShapeMax;  // 8 x 32 covers up to 256 frames
             if (batch_fast_ok(j.stack, prep) && shape >= 0) {
                 Group* g = nullptr;
+                const bool absolute = prep.k.th.absolute != 0;
                 for (auto& gg : b->groups)
-                    if (gg.bps == j.stack->bps && gg.channels == j.stack->channels) g = &gg;
+                    if (gg.bps == j.stack->bps && gg.channels == j.stack->channels && gg.shape == shape &&
+                        gg.absolute == absolute)
+                        g = &gg;
                 if (!g) {
                     b->groups.emplace_back();
                     g = &b->groups.back();
                     g->bps = j.stack->bps;
                     g->channels = j.stack->channels;
+                    g->shape = shape;
+                    g->absolute = absolute;
                 }
                 BatchJobK bj{};
                 bj.k = prep.k;

@@ -1,12 +1,12 @@
-// k_batch.h -- many outlier composites in one persistent launch.
+// k_batch.h -- many outlier composites in a few persistent launches.
 //
-// Every job (a stack, its frame selection and options) is cut into row tiles of its own
-// tile shape; the jobs' tiles form one work list, biggest jobs first.  A grid sized to the
-// device's resident capacity loops: one lane of each workgroup takes the next item from a
-// device counter, the workgroup stages that job's frame-offset table in LDS when the job
-// changes, and runs the same tile body as the single-stack kernel (outlier_tile), selected
-// per job from the shapes with F <= 32 and both threshold kinds.  Results are bit-identical
-// to one launch per stack: the per-pixel arithmetic is the same code.
+// Jobs (a stack, its frame selection and options) are grouped by tile shape and threshold
+// kind; each group is one persistent launch.  Every job is cut into row tiles; a group's
+// tiles form one work list, biggest jobs first.  A grid sized to the device's resident
+// capacity loops: one lane of each workgroup takes the next item from a device counter, the
+// workgroup stages that job's frame-offset table in LDS when the job changes, and runs the
+// same tile body as the single-stack kernel (outlier_tile).  Results are bit-identical to
+// one launch per stack: the per-pixel arithmetic is the same code.
 #pragma once
 #include "kernels_fast.h"
 
@@ -20,104 +20,88 @@ struct BatchJobK {
     uint32_t pad;
 };
 
-template <int B, int C, int SH, bool ABS>
-__device__ __forceinline__ void batch_tile(const OutlierK& k, uint32_t bx, uint32_t y, unsigned char* smem) {
-    constexpr int G = kShapes[SH].G, F = kShapes[SH].F;
-    outlier_tile<B, C, G, F, ABS>(k, bx, y, *reinterpret_cast<TileShared<C, G, F>*>(smem));
-}
-
-// Frame positions covered by a tile of shape sh (G * F).
-__device__ __forceinline__ uint32_t shape_frames(int sh) {
-    constexpr uint32_t gf[kBatchShapeMax + 1] = {
-        kShapes[0].G * kShapes[0].F, kShapes[1].G * kShapes[1].F, kShapes[2].G * kShapes[2].F,
-        kShapes[3].G * kShapes[3].F, kShapes[4].G * kShapes[4].F, kShapes[5].G * kShapes[5].F,
-        kShapes[6].G * kShapes[6].F, kShapes[7].G * kShapes[7].F};
-    return gf[sh];
-}
-
-// LDS of the largest tile shape the batch kernel carries.
-template <int C, int SH>
-constexpr size_t batch_smem() {
-    if constexpr (SH < 0) return 0;
-    else {
-        constexpr size_t a = sizeof(TileShared<C, kShapes[SH].G, kShapes[SH].F>);
-        constexpr size_t b = batch_smem<C, SH - 1>();
-        return a > b ? a : b;
-    }
-}
-
 template <int B>
 __global__ void batch_reset_kernel(unsigned int* counter) { *counter = 0u; }
 
-template <int B, int C>
+// All jobs of one launch share the sample width B, the channel count C, the tile shape SH and
+// the threshold kind ABS, so the kernel is the single-stack tile body with its own register
+// budget.
+template <int B, int C, int SH, bool ABS>
 __global__ void __launch_bounds__(256) outlier_batch_kernel(const BatchJobK* __restrict__ jobs,
                                                             uint32_t njobs, uint32_t total,
                                                             unsigned int* __restrict__ counter) {
-    // every TileShared starts with its frame-offset table
-    __shared__ __align__(16) unsigned char smem[batch_smem<C, kBatchShapeMax>()];
-    uint64_t* const s_off = reinterpret_cast<uint64_t*>(smem);
+    constexpr int G = kShapes[SH].G, F = kShapes[SH].F;
+    __shared__ TileShared<C, G, F> sh;
     __shared__ uint32_t s_item;
     uint32_t cur = 0xFFFFFFFFu;
     for (;;) {
         if (threadIdx.x == 0) s_item = atomicAdd(counter, 1u);
         __syncthreads();
-        const uint32_t item = s_item;
-        if (item >= total) break;  // every lane of the workgroup sees the same item
+        // uniform across the workgroup; readfirstlane lets the job record live in SGPRs
+        const uint32_t item = __builtin_amdgcn_readfirstlane(s_item);
+        if (item >= total) break;
         uint32_t lo = 0, hi = njobs;  // last job whose first item is <= item
         while (hi - lo > 1) {
             const uint32_t mid = (lo + hi) >> 1;
             if (jobs[mid].item0 <= item) lo = mid;
             else hi = mid;
         }
+        lo = __builtin_amdgcn_readfirstlane(lo);
         const BatchJobK& J = jobs[lo];
         if (lo != cur) {
-            const uint32_t nf = shape_frames(J.shape);
-            load_offsets<B>(s_off, nf, J.k.pos_off, J.k.n);
+            load_offsets<B>(sh.off, G * F, J.k.pos_off, J.k.n);
             cur = lo;
         }
         __syncthreads();
         const uint32_t tile = item - J.item0;
-        const uint32_t bx = tile % J.tiles_x, y = tile / J.tiles_x;
-        switch (J.shape * 2 + (J.k.th.absolute ? 1 : 0)) {
-        case 0: batch_tile<B, C, 0, false>(J.k, bx, y, smem); break;
-        case 1: batch_tile<B, C, 0, true>(J.k, bx, y, smem); break;
-        case 2: batch_tile<B, C, 1, false>(J.k, bx, y, smem); break;
-        case 3: batch_tile<B, C, 1, true>(J.k, bx, y, smem); break;
-        case 4: batch_tile<B, C, 2, false>(J.k, bx, y, smem); break;
-        case 5: batch_tile<B, C, 2, true>(J.k, bx, y, smem); break;
-        case 6: batch_tile<B, C, 3, false>(J.k, bx, y, smem); break;
-        case 7: batch_tile<B, C, 3, true>(J.k, bx, y, smem); break;
-        case 8: batch_tile<B, C, 4, false>(J.k, bx, y, smem); break;
-        case 9: batch_tile<B, C, 4, true>(J.k, bx, y, smem); break;
-        case 10: batch_tile<B, C, 5, false>(J.k, bx, y, smem); break;
-        case 11: batch_tile<B, C, 5, true>(J.k, bx, y, smem); break;
-        case 12: batch_tile<B, C, 6, false>(J.k, bx, y, smem); break;
-        case 13: batch_tile<B, C, 6, true>(J.k, bx, y, smem); break;
-        case 14: batch_tile<B, C, 7, false>(J.k, bx, y, smem); break;
-        default: batch_tile<B, C, 7, true>(J.k, bx, y, smem); break;
-        }
+        outlier_tile<B, C, G, F, ABS>(J.k, tile % J.tiles_x, tile / J.tiles_x, sh);
         __syncthreads();  // the tile's LDS and s_item are rewritten next iteration
     }
 }
 
-// Defined per channel count for 8-bit stacks in kernels_batch_b1_c<C>.hip.
-int launch_batch_b1_c3(const BatchJobK* jobs, uint32_t njobs, uint32_t total, unsigned int* counter,
-                       int cus, hipStream_t s);
-int launch_batch_b1_c4(const BatchJobK* jobs, uint32_t njobs, uint32_t total, unsigned int* counter,
-                       int cus, hipStream_t s);
-
-template <int B, int C>
-inline int launch_batch_bc(const BatchJobK* jobs, uint32_t njobs, uint32_t total, unsigned int* counter,
-                           int cus, hipStream_t s) {
+template <int B, int C, int SH, bool ABS>
+inline int launch_batch_one(const BatchJobK* jobs, uint32_t njobs, uint32_t total, unsigned int* counter,
+                            int cus, hipStream_t s) {
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, outlier_batch_kernel<B, C>, 256, 0) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, outlier_batch_kernel<B, C, SH, ABS>, 256, 0) !=
             hipSuccess ||
         per_cu < 1)
         per_cu = 1;
     const uint32_t grid = (uint32_t)(cus > 0 ? cus : 256) * (uint32_t)per_cu;
     hipLaunchKernelGGL(batch_reset_kernel<B>, dim3(1), dim3(1), 0, s, counter);
-    hipLaunchKernelGGL((outlier_batch_kernel<B, C>), dim3(grid), dim3(256), 0, s, jobs, njobs, total, counter);
+    hipLaunchKernelGGL((outlier_batch_kernel<B, C, SH, ABS>), dim3(grid), dim3(256), 0, s, jobs, njobs, total,
+                       counter);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
+
+template <int B, int C>
+inline int launch_batch_bc(const BatchJobK* jobs, uint32_t njobs, uint32_t total, unsigned int* counter,
+                           int shape, bool absolute, int cus, hipStream_t s) {
+    switch (shape * 2 + (absolute ? 1 : 0)) {
+    case 0: return launch_batch_one<B, C, 0, false>(jobs, njobs, total, counter, cus, s);
+    case 1: return launch_batch_one<B, C, 0, true>(jobs, njobs, total, counter, cus, s);
+    case 2: return launch_batch_one<B, C, 1, false>(jobs, njobs, total, counter, cus, s);
+    case 3: return launch_batch_one<B, C, 1, true>(jobs, njobs, total, counter, cus, s);
+    case 4: return launch_batch_one<B, C, 2, false>(jobs, njobs, total, counter, cus, s);
+    case 5: return launch_batch_one<B, C, 2, true>(jobs, njobs, total, counter, cus, s);
+    case 6: return launch_batch_one<B, C, 3, false>(jobs, njobs, total, counter, cus, s);
+    case 7: return launch_batch_one<B, C, 3, true>(jobs, njobs, total, counter, cus, s);
+    case 8: return launch_batch_one<B, C, 4, false>(jobs, njobs, total, counter, cus, s);
+    case 9: return launch_batch_one<B, C, 4, true>(jobs, njobs, total, counter, cus, s);
+    case 10: return launch_batch_one<B, C, 5, false>(jobs, njobs, total, counter, cus, s);
+    case 11: return launch_batch_one<B, C, 5, true>(jobs, njobs, total, counter, cus, s);
+    case 12: return launch_batch_one<B, C, 6, false>(jobs, njobs, total, counter, cus, s);
+    case 13: return launch_batch_one<B, C, 6, true>(jobs, njobs, total, counter, cus, s);
+    case 14: return launch_batch_one<B, C, 7, false>(jobs, njobs, total, counter, cus, s);
+    case 15: return launch_batch_one<B, C, 7, true>(jobs, njobs, total, counter, cus, s);
+    default: return -1;
+    }
+}
+
+// Defined per channel count for 8-bit stacks in kernels_batch_b1_c<C>.hip.
+int launch_batch_b1_c3(const BatchJobK* jobs, uint32_t njobs, uint32_t total, unsigned int* counter, int shape,
+                       bool absolute, int cus, hipStream_t s);
+int launch_batch_b1_c4(const BatchJobK* jobs, uint32_t njobs, uint32_t total, unsigned int* counter, int shape,
+                       bool absolute, int cus, hipStream_t s);
 
 }  // namespace cp

@@ -40,9 +40,6 @@ typedef unsigned short us2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ us2 as_us2(uint32_t v) { return __builtin_bit_cast(us2, v); }
 __device__ __forceinline__ uint32_t as_u32(us2 v) { return __builtin_bit_cast(uint32_t, v); }
 
-// Byte 0 of v in all four bytes.
-__device__ __forceinline__ uint32_t rep4(uint32_t v) { return __builtin_amdgcn_perm(0u, v, 0u); }
-
 // Sum / min / max over the G adjacent lanes of a pixel group (G in {1,2,4,8}); the DPP
 // moves fold into v_add/v_min/v_max_u32_dpp.  Every lane of the group receives the result.
 template <int G>
@@ -69,54 +66,56 @@ __device__ __forceinline__ uint32_t grp_max_u32(uint32_t v) {
     return v;
 }
 
-// #{bytes < v} over the group, v in [1, 255].
-template <int G, int ND>
-__device__ __forceinline__ uint32_t byte_count_lt(const uint32_t (&y)[ND], uint32_t v, int32_t ntot) {
-    const uint32_t V = rep4(v), Vm = V - 0x01010101u;
-    uint32_t a = 0, b = 0;
-#pragma unroll
-    for (int i = 0; i < ND; ++i) {
-        a = __builtin_amdgcn_sad_u8(y[i], V, a);
-        b = __builtin_amdgcn_sad_u8(y[i], Vm, b);
-    }
-    return (uint32_t)(ntot + grp_sum<G>((int32_t)(a - b))) >> 1;
-}
-
 // For every channel c and rank k[r]: the largest v in [0, 255] with #{bytes < v} <= k[r]
 // over the group's ND dwords per lane of y[c].  Candidates are kept replicated in all four
 // bytes, so each step is two v_sad_u8 per dword, a G-lane reduction and a select per
-// (channel, rank); the C*R chains are independent and interleave.
-template <int G, int ND, int C, int R>
+// (channel, rank); the C*R chains are independent and interleave.  With CNT, cnt[c][r]
+// receives #{bytes <= v}: v + 1 is the last rejected candidate (v = prefix 0 1..1 and the
+// step at that 0 bit tested prefix 1 0..0), whose count the search already computed; no
+// rejection means v = 255 and every byte counts.
+template <int G, int ND, int C, int R, bool CNT>
 __device__ __forceinline__ void byte_select_multi(const uint32_t (&y)[C][ND], const uint32_t (&k)[R],
-                                                  int32_t ntot, uint32_t (&out)[C][R]) {
+                                                  int32_t ntot, uint32_t (&out)[C][R], uint32_t (&cnt)[C][R]) {
     uint32_t lo[C][R];
+    int32_t drej[C][R], thr[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) thr[r] = 2 * (int32_t)k[r] - ntot;
 #pragma unroll
     for (int c = 0; c < C; ++c)
 #pragma unroll
-        for (int r = 0; r < R; ++r) lo[c][r] = 0;
+        for (int r = 0; r < R; ++r) {
+            lo[c][r] = 0;
+            drej[c][r] = ntot;
+        }
 #pragma unroll 1
     for (int bit = 7; bit >= 0; --bit) {
         const uint32_t inc = 0x01010101u << bit;
+        const uint32_t below = inc - 0x01010101u;  // the lower bits of every byte
 #pragma unroll
         for (int c = 0; c < C; ++c) {
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                const uint32_t V = lo[c][r] | inc, Vm = V - 0x01010101u;
+                // lo has no bits at or below `bit`, so V - 1 (per byte) is lo | below
+                const uint32_t V = lo[c][r] | inc, Vm = lo[c][r] | below;
                 uint32_t a = 0, b = 0;
 #pragma unroll
                 for (int i = 0; i < ND; ++i) {
                     a = __builtin_amdgcn_sad_u8(y[c][i], V, a);
                     b = __builtin_amdgcn_sad_u8(y[c][i], Vm, b);
                 }
-                // ntot + d = 2 #{y < v}
-                if (ntot + grp_sum<G>((int32_t)(a - b)) <= 2 * (int32_t)k[r]) lo[c][r] = V;
+                const int32_t d = grp_sum<G>((int32_t)(a - b));  // 2 #{y < v} - ntot
+                if (d <= thr[r]) lo[c][r] = V;
+                else if constexpr (CNT) drej[c][r] = d;
             }
         }
     }
 #pragma unroll
     for (int c = 0; c < C; ++c)
 #pragma unroll
-        for (int r = 0; r < R; ++r) out[c][r] = lo[c][r] & 0xFFu;
+        for (int r = 0; r < R; ++r) {
+            out[c][r] = lo[c][r] & 0xFFu;
+            if constexpr (CNT) cnt[c][r] = (uint32_t)(drej[c][r] + ntot) >> 1;
+        }
 }
 
 // Low bytes of the 16-bit samples (8-bit stacks keep one sample per 16-bit half).
@@ -150,11 +149,11 @@ __device__ __forceinline__ void select_rank_u16_multi(const uint32_t (&raw)[C][N
 #pragma unroll
         for (int i = 0; i < ND; ++i) y[c][i] = __builtin_amdgcn_perm(raw[c][2 * i + 1], raw[c][2 * i], 0x07050301u);
     const uint32_t kk[1] = {k};
-    uint32_t H[C][1], L[C][1];
-    byte_select_multi<G, ND, C, 1>(y, kk, ntot, H);
+    uint32_t H[C][1], L[C][1], unused[C][1];
+    byte_select_multi<G, ND, C, 1, false>(y, kk, ntot, H, unused);
 #pragma unroll
     for (int c = 0; c < C; ++c) pack_window<NW>(raw[c], H[c][0] << 8, y[c]);
-    byte_select_multi<G, ND, C, 1>(y, kk, ntot, L);
+    byte_select_multi<G, ND, C, 1, false>(y, kk, ntot, L, unused);
 #pragma unroll
     for (int c = 0; c < C; ++c) v[c] = (H[c][0] << 8) | L[c][0];
 }
@@ -233,18 +232,16 @@ __device__ __forceinline__ void rank_pairs(const uint32_t (&raw)[C][NW], uint32_
     constexpr int ND = NW / 2;
     constexpr uint32_t MAXV = B == 1 ? 255u : 65535u;
     const uint32_t kk[1] = {r0};
-    uint32_t sel[C][1];
+    uint32_t sel[C][1], cle[C][1];
     if constexpr (B == 1) {
         uint32_t y[C][ND];
 #pragma unroll
         for (int c = 0; c < C; ++c) pack_low_bytes<NW>(raw[c], y[c]);
-        byte_select_multi<G, ND, C, 1>(y, kk, ntot, sel);
+        byte_select_multi<G, ND, C, 1, true>(y, kk, ntot, sel, cle);
 #pragma unroll
         for (int c = 0; c < C; ++c) {
             v0[c] = sel[c][0];
-            if (r1 == r0) { v1[c] = v0[c]; continue; }
-            const uint32_t cle = v0[c] >= 255u ? (uint32_t)ntot : byte_count_lt<G, ND>(y[c], v0[c] + 1, ntot);
-            v1[c] = neighbour<G, NW>(raw[c], v0[c], cle, r1);
+            v1[c] = r1 == r0 ? v0[c] : neighbour<G, NW>(raw[c], v0[c], cle[c][0], r1);
         }
     } else if constexpr (WINDOW) {
         uint32_t y[C][ND], lo[C];
@@ -254,14 +251,13 @@ __device__ __forceinline__ void rank_pairs(const uint32_t (&raw)[C][NW], uint32_
             lo[c] = min(est >= 127u ? est - 127u : 0u, 65535u - 255u);
             pack_window<NW>(raw[c], lo[c], y[c]);
         }
-        byte_select_multi<G, ND, C, 1>(y, kk, ntot, sel);
+        byte_select_multi<G, ND, C, 1, true>(y, kk, ntot, sel, cle);
 #pragma unroll
         for (int c = 0; c < C; ++c) {
             const uint32_t ys = sel[c][0];
-            uint32_t cle = 0;
+            uint32_t le = cle[c][0];  // #{x <= v0} when v0 lies inside the window
             if (ys >= 1u && ys <= 254u) {
                 v0[c] = lo[c] + ys;
-                if (r1 != r0) cle = byte_count_lt<G, ND>(y[c], ys + 1, ntot);  // #{x < v0 + 1}
             } else {
                 uint32_t one[1][NW];
 #pragma unroll
@@ -269,9 +265,9 @@ __device__ __forceinline__ void rank_pairs(const uint32_t (&raw)[C][NW], uint32_
                 uint32_t v[1];
                 select_rank_u16_multi<G, NW, 1>(one, r0, ntot, v);
                 v0[c] = v[0];
-                if (r1 != r0) cle = v0[c] >= MAXV ? (uint32_t)ntot : count_le<G, NW>(raw[c], v0[c], ntot);
+                if (r1 != r0) le = v0[c] >= MAXV ? (uint32_t)ntot : count_le<G, NW>(raw[c], v0[c], ntot);
             }
-            v1[c] = r1 == r0 ? v0[c] : neighbour<G, NW>(raw[c], v0[c], cle, r1);
+            v1[c] = r1 == r0 ? v0[c] : neighbour<G, NW>(raw[c], v0[c], le, r1);
         }
     } else {
         select_rank_u16_multi<G, NW, C>(raw, r0, ntot, v0);
@@ -423,35 +419,20 @@ __device__ __forceinline__ void outlier_tile(const OutlierK& k, uint32_t bx, uin
         }
 
         // ---- medians / inverse IQR per channel
-        const RankPlan plan = make_rank_plan(N, ABS);
+        const RankPlan plan = rank_plan_core(N, ABS);
         float med[4] = {0, 0, 0, 0}, iqr_inv[4] = {0, 0, 0, 0};
         {
-            uint32_t m0[C], m1[C];
+            uint32_t m0[C], m1[C], a0[C], a1[C], b0[C], b1[C];
             rank_pairs<B, G, NW, C, ABS>(raw, plan.m0, plan.m1, ntot, m0, m1);
-            uint32_t a0[C], a1[C], b0[C], b1[C];
             if constexpr (!ABS) {
                 rank_pairs<B, G, NW, C, false>(raw, plan.a0, plan.a1, ntot, a0, a1);
                 rank_pairs<B, G, NW, C, false>(raw, plan.b0, plan.b1, ntot, b0, b1);
             }
 #pragma unroll
             for (int c = 0; c < C; ++c) {
-                uint32_t vals[6] = {0, 0, 0, 0, 0, 0};
-#pragma unroll
-                for (int i = 0; i < 6; ++i) {  // equal ranks carry equal values
-                    const uint32_t rk = plan.rank[i];
-                    uint32_t v = 0;
-                    if constexpr (!ABS) {
-                        if (rk == plan.b1) v = b1[c];
-                        if (rk == plan.b0) v = b0[c];
-                        if (rk == plan.a1) v = a1[c];
-                        if (rk == plan.a0) v = a0[c];
-                    }
-                    if (rk == plan.m1) v = m1[c];
-                    if (rk == plan.m0) v = m0[c];
-                    vals[i] = v;
-                }
                 float mc, ic;
-                stats_from_ranks(plan, vals, ABS, &mc, &ic);
+                if constexpr (ABS) stats_from_values(plan, m0[c], m1[c], 0, 0, 0, 0, true, &mc, &ic);
+                else stats_from_values(plan, m0[c], m1[c], a0[c], a1[c], b0[c], b1[c], false, &mc, &ic);
                 // channels without weight keep zero statistics
                 med[c] = k.w[c] == 0.0f ? 0.0f : mc;
                 iqr_inv[c] = k.w[c] == 0.0f ? 0.0f : ic;
@@ -734,26 +715,32 @@ __device__ __forceinline__ void outlier_tile(const OutlierK& k, uint32_t bx, uin
             float blend_inv = 1.0f;
 #pragma unroll
             for (int c = 0; c < C; ++c) pix_new[c] = (float)pixel[c];
-            const bool fwd = k.outlier == 4;
+            // outliers in frame order (forward) or reverse order (backward); the direction is a
+            // template constant so that every mask word is a fixed register
+            auto walk = [&](auto fwd_t) {
+                constexpr bool FWD = decltype(fwd_t)::value;
 #pragma unroll
-            for (int qq = 0; qq < G; ++qq) {
-                const int q = fwd ? qq : G - 1 - qq;
+                for (int qq = 0; qq < G; ++qq) {
+                    const int q = FWD ? qq : G - 1 - qq;
 #pragma unroll
-                for (int ww = 0; ww < MW; ++ww) {
-                    const int w = fwd ? ww : MW - 1 - ww;
-                    uint32_t mm = gm[q][w];
-                    while (mm) {
-                        const uint32_t b = fwd ? (uint32_t)__builtin_ctz(mm) : 31 - __builtin_clz(mm);
-                        mm &= ~(1u << b);
-                        const uint32_t t = (uint32_t)q * F + (uint32_t)w * 32 + b;
-                        const float d = dist_of(t, s);
-                        const float fade = fade_pick(k.fade, (int32_t)t, (int32_t)N, k.frame_offset);
-                        const float blend = fade * blend_value(k.th, sqrtf(d));
-                        blend_into_f32<C>(pix_new, s, blend);
-                        blend_inv *= 1.0f - blend;
+                    for (int ww = 0; ww < MW; ++ww) {
+                        const int w = FWD ? ww : MW - 1 - ww;
+                        uint32_t mm = gm[q][w];
+                        while (mm) {
+                            const uint32_t b = FWD ? (uint32_t)__builtin_ctz(mm) : 31 - __builtin_clz(mm);
+                            mm &= ~(1u << b);
+                            const uint32_t t = (uint32_t)q * F + (uint32_t)w * 32 + b;
+                            const float d = dist_of(t, s);
+                            const float fade = fade_pick(k.fade, (int32_t)t, (int32_t)N, k.frame_offset);
+                            const float blend = fade * blend_value(k.th, sqrtf(d));
+                            blend_into_f32<C>(pix_new, s, blend);
+                            blend_inv *= 1.0f - blend;
+                        }
                     }
                 }
-            }
+            };
+            if (k.outlier == 4) walk(std::true_type{});
+            else walk(std::false_type{});
 #pragma unroll
             for (int c = 0; c < C; ++c) pixel[c] = sat_cast(roundf(pix_new[c]), maxv);
             blend_byte = sat_cast(roundf((1.0f - blend_inv) * 255.0f), 255u);

@@ -37,6 +37,18 @@ __host__ __device__ inline void quantile_plan(uint32_t len, float q, uint32_t* r
     else *mode = 2;
 }
 
+// The ranks and interpolation weights alone (no rank list).
+__host__ __device__ inline RankPlan rank_plan_core(uint32_t len, bool absolute) {
+    RankPlan p{};
+    if ((len + 1) % 2 == 0) { p.m0 = p.m1 = (len + 1) / 2 - 1; }
+    else { p.m1 = (len + 1) / 2; p.m0 = p.m1 - 1; }
+    if (!absolute) {
+        quantile_plan(len, 0.25f, &p.a0, &p.a1, &p.fa, &p.qa_mode);
+        quantile_plan(len, 0.75f, &p.b0, &p.b1, &p.fb, &p.qb_mode);
+    }
+    return p;
+}
+
 __host__ __device__ inline RankPlan make_rank_plan(uint32_t len, bool absolute) {
     RankPlan p{};
     if ((len + 1) % 2 == 0) { p.m0 = p.m1 = (len + 1) / 2 - 1; }
@@ -79,6 +91,20 @@ __device__ __forceinline__ void stats_from_ranks(const RankPlan& p, const uint32
     };
     float q1 = quant(p.a0, p.a1, p.fa, p.qa_mode);
     float q3 = quant(p.b0, p.b1, p.fb, p.qb_mode);
+    float iqr = q3 - q1;
+    if (iqr == 0.0f) iqr = 1.0f;
+    *iqr_inv = 1.0f / iqr;
+}
+
+// stats_from_ranks with the values of ranks m0, m1, a0, a1, b0, b1 given directly.
+__device__ __forceinline__ void stats_from_values(const RankPlan& p, uint32_t m0, uint32_t m1, uint32_t a0,
+                                                  uint32_t a1, uint32_t b0, uint32_t b1, bool absolute,
+                                                  float* median, float* iqr_inv) {
+    float d0 = (float)m0, d1 = (float)m1;
+    *median = p.m0 == p.m1 ? d0 : 0.5f * (d0 + d1);
+    if (absolute) { *iqr_inv = 0.0f; return; }
+    const float q1 = p.qa_mode == 2 ? (1.0f - p.fa) * (float)a0 + p.fa * (float)a1 : (float)a0;
+    const float q3 = p.qb_mode == 2 ? (1.0f - p.fb) * (float)b0 + p.fb * (float)b1 : (float)b0;
     float iqr = q3 - q1;
     if (iqr == 0.0f) iqr = 1.0f;
     *iqr_inv = 1.0f / iqr;

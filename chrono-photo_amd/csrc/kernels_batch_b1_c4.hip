@@ -3,9 +3,9 @@
 
 namespace cp {
 
-int launch_batch_b1_c4(const BatchJobK* jobs, uint32_t njobs, uint32_t total, unsigned int* counter,
-                       int cus, hipStream_t s) {
-    return launch_batch_bc<1, 4>(jobs, njobs, total, counter, cus, s);
+int launch_batch_b1_c4(const BatchJobK* jobs, uint32_t njobs, uint32_t total, unsigned int* counter, int shape,
+                       bool absolute, int cus, hipStream_t s) {
+    return launch_batch_bc<1, 4>(jobs, njobs, total, counter, shape, absolute, cus, s);
 }
 
 }  // namespace cp
