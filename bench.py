@@ -64,7 +64,7 @@ def _stack_from_scene(cpa, ctx, sc):
 
 def run_batching(cpa, H, ctx, seed=3, n_stacks=48):
     """48 independent RGB8 stacks (frames U{8..200}, 640x480..1920x1080, abs/rel thresholds
-    alternating, the six --outlier modes round-robin): one persistent graph launch vs one
+    alternating, the six --outlier modes round-robin): one graph of per-group launches vs one
     launch per stack."""
     import numpy as np
     rng = np.random.default_rng(seed)
@@ -120,7 +120,7 @@ def run_batching(cpa, H, ctx, seed=3, n_stacks=48):
             "plain_value": total / plain_s, "batched_value": total / batch_s,
             "speedup": plain_s / batch_s, "byte_identical": bool(same),
             "plain": "48 sequential cp_outlier launches, one stack each, each waited for",
-            "batched": "one persistent graph launch (cp_batch_create/cp_batch_launch)"}
+            "batched": "one graph replay: one launch per tile-shape group (cp_batch_create/cp_batch_launch)"}
 
 
 def run_serving(cpa, H, ctx, seed=4, n_jobs=24, window=60, n_video=600):

@@ -801,7 +801,7 @@ def _job_array(jobs):
 
 
 class Batch:
-    """cp_batch: many composites in one persistent launch, captured in a HIP graph."""
+    """cp_batch: many composites in a few launches (one per tile-shape group), captured in a HIP graph."""
 
     def __init__(self, ctx: Context, jobs, graph: bool = True):
         self.ctx = ctx

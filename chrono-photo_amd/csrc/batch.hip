@@ -31,7 +31,6 @@ struct Group {
     std::vector<uint64_t> cost;
     uint32_t total = 0;
     BatchJobK* d_jobs = nullptr;
-    unsigned int* d_counter = nullptr;
 };
 
 bool batch_fast_ok(const cp_stack* st, const OutlierPrep& p) {
@@ -42,10 +41,10 @@ bool batch_fast_ok(const cp_stack* st, const OutlierPrep& p) {
     return p.k.n <= 256;
 }
 
-int launch_group(const Group& g, int cus, hipStream_t s) {
+int launch_group(const Group& g, hipStream_t s) {
     const uint32_t nj = (uint32_t)g.jobs.size();
-    if (g.channels == 3) return launch_batch_b1_c3(g.d_jobs, nj, g.total, g.d_counter, g.shape, g.absolute, cus, s);
-    return launch_batch_b1_c4(g.d_jobs, nj, g.total, g.d_counter, g.shape, g.absolute, cus, s);
+    if (g.channels == 3) return launch_batch_b1_c3(g.d_jobs, nj, g.total, g.shape, g.absolute, s);
+    return launch_batch_b1_c4(g.d_jobs, nj, g.total, g.shape, g.absolute, s);
 }
 
 }  // namespace
@@ -178,14 +177,15 @@ extern "C" CP_API int cp_batch_create(cp_ctx* ctx, const cp_job* jobs, uint32_t 
         g.jobs.swap(jobs_sorted);
         g.pos.swap(pos_sorted);
         g.fades.swap(fade_sorted);
-        uint64_t item = 0;
+        uint64_t tiles = 0;
         for (auto& bj : g.jobs) {
-            bj.item0 = (uint32_t)item;
-            item += (uint64_t)bj.tiles_x * bj.k.out_h;
+            bj.tile0 = (uint32_t)tiles;
+            bj.tiles = bj.tiles_x * bj.k.out_h;
+            tiles += bj.tiles;
         }
-        if (item >= 0xFFFFFFFFull) return fail(CP_ERR_CAPACITY, "batch has too many tiles");
-        g.total = (uint32_t)item;
-        bytes += align(sizeof(BatchJobK) * g.jobs.size()) + 256;
+        if (tiles >= 0x7FFFFFFFull) return fail(CP_ERR_CAPACITY, "batch has too many tiles");
+        g.total = (uint32_t)tiles;
+        bytes += align(sizeof(BatchJobK) * g.jobs.size());
         for (auto& p : g.pos) bytes += align(8 * p.size());
         for (auto& f : g.fades) bytes += align(4 * f.size());
     }
@@ -204,8 +204,6 @@ extern "C" CP_API int cp_batch_create(cp_ctx* ctx, const cp_job* jobs, uint32_t 
         for (auto& g : b->groups) {
             const uint64_t jobs_off = off;
             off += align(sizeof(BatchJobK) * g.jobs.size());
-            const uint64_t counter_off = off;
-            off += 256;
             for (size_t i = 0; i < g.jobs.size(); ++i) {
                 memcpy(host.data() + off, g.pos[i].data(), 8 * g.pos[i].size());
                 g.jobs[i].k.pos_off = (const uint64_t*)(dbase + off);
@@ -218,7 +216,6 @@ extern "C" CP_API int cp_batch_create(cp_ctx* ctx, const cp_job* jobs, uint32_t 
             }
             memcpy(host.data() + jobs_off, g.jobs.data(), sizeof(BatchJobK) * g.jobs.size());
             g.d_jobs = (BatchJobK*)(dbase + jobs_off);
-            g.d_counter = (unsigned int*)(dbase + counter_off);
         }
         CP_HIP_TRY(hipMemcpy(b->d_tables, host.data(), bytes, hipMemcpyHostToDevice));
     }
@@ -226,7 +223,7 @@ extern "C" CP_API int cp_batch_create(cp_ctx* ctx, const cp_job* jobs, uint32_t 
         std::lock_guard<std::mutex> lock(ctx->mu);
         CP_HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
         int lrc = 0;
-        for (auto& g : b->groups) lrc |= launch_group(g, ctx->cu_count, ctx->stream);
+        for (auto& g : b->groups) lrc |= launch_group(g, ctx->stream);
         hipGraph_t graph = nullptr;
         hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
         if (lrc || e != hipSuccess || !graph)
@@ -248,7 +245,7 @@ extern "C" CP_API int cp_batch_launch(cp_batch* b) {
             CP_HIP_TRY(hipGraphLaunch(b->exec, ctx->stream));
         } else {
             for (auto& g : b->groups)
-                if (launch_group(g, ctx->cu_count, ctx->stream))
+                if (launch_group(g, ctx->stream))
                     return fail(CP_ERR_HIP, "batch kernel launch: %s", hipGetErrorString(hipGetLastError()));
         }
     }

@@ -296,41 +296,50 @@ __device__ __forceinline__ float frame_dist(const uint32_t (&s)[4], const float 
 
 // Per-tile LDS: the frame-offset table, every lane's classification summary and every
 // pixel's order statistics.
+// Layouts keep every access of a wave in distinct LDS banks: the offset table leaves two
+// spare entries after each lane block, so the G lanes of a pixel (which read entries
+// g*F + j at once) start in different banks; per-lane summaries sit at slot(g, p), which
+// rotates each frame block's row by 4 so that the finishing lanes (fixed g, consecutive p)
+// and the writing lanes (8 pixels x G blocks) both spread over the banks; per-pixel values
+// are stored channel-major.
 template <int C, int G, int F>
 struct TileShared {
     static constexpr int P = 256 / G;
     static constexpr int MW = F > 32 ? 2 : 1;  // mask words per lane
-    struct Pix {
-        float med[4];
-        float iqr_inv[4];
-        uint32_t s0[4];       // frame 0 samples
-        uint32_t sum_all[4];  // average modes
-        uint32_t sum_out[4];
-    };
-    uint64_t off[G * F];       // frame byte offsets; positions >= N repeat position N-1
-    uint32_t mask[256 * MW];   // lane l, word w: outlier bits of frames t0 + 32w + b
-    float lmax[256];           // lane's largest outlier distance (valid when lmaxi != ~0)
-    uint32_t lmaxi[256];       // its first frame
-    Pix px[P];
+    static constexpr int OS = F + 2;           // offset-table stride of a lane block
+    uint64_t off[G * OS];      // frame byte offsets at oidx(t); positions >= N repeat N-1
+    uint32_t mask[MW][256];    // [w][slot]: outlier bits of frames g*F + 32w + b
+    float lmax[256];           // [slot]: the lane's largest outlier distance (valid when lmaxi != ~0)
+    uint32_t lmaxi[256];       // [slot]: its first frame
+    float med[4][P];
+    float iqr_inv[4][P];
+    uint32_t s0[4][P];       // frame 0 samples
+    uint32_t sum_all[4][P];  // average modes
+    uint32_t sum_out[4][P];
+    __device__ static __forceinline__ uint32_t oidx(uint32_t t) { return t + 2u * (t / (uint32_t)F); }
+    __device__ static __forceinline__ uint32_t slot(uint32_t g, uint32_t p) {
+        return g * (uint32_t)P + ((p + 4u * g) & (uint32_t)(P - 1));
+    }
 };
 
 // Fills the frame-offset table of a tile (all 256 lanes take part): byte offsets of the
-// frame positions, the last one repeated up to G*F entries.
-template <int B>
+// frame positions, the last one repeated up to `count` positions, at oidx(t) of a table
+// with blocks of F entries.
+template <int B, int F>
 __device__ __forceinline__ void load_offsets(uint64_t* off, uint32_t count, const uint64_t* __restrict__ pos_off,
                                              uint32_t n) {
-    for (uint32_t i = threadIdx.x; i < count; i += 256) off[i] = pos_off[min(i, n - 1)] * B;
+    for (uint32_t i = threadIdx.x; i < count; i += 256) off[i + 2u * (i / (uint32_t)F)] = pos_off[min(i, n - 1)] * B;
 }
 
-// Loads frames [t0, t0 + F) of one pixel into raw[c][j/2] (frame j in half j&1); base is
-// the pixel's address in stack slot 0 and off holds the frame byte offsets.
+// Loads the F frames of one lane block into raw[c][j/2] (frame j in half j&1); base is the
+// pixel's address in stack slot 0 and off the block's F frame byte offsets.
 template <int B, int C, int F>
-__device__ __forceinline__ void load_block(const uint8_t* __restrict__ base, const uint64_t* off, uint32_t t0,
+__device__ __forceinline__ void load_block(const uint8_t* __restrict__ base, const uint64_t* off,
                                            uint32_t (&raw)[C][F / 2]) {
 #pragma unroll
     for (int i = 0; i < F / 2; ++i) {
-        const uint8_t* p0 = base + off[t0 + 2 * i];
-        const uint8_t* p1 = base + off[t0 + 2 * i + 1];
+        const uint8_t* p0 = base + off[2 * i];
+        const uint8_t* p1 = base + off[2 * i + 1];
         if constexpr (B == 2 && C == 3) {
             uint32_t d0, d1;  // channels 0-1 of each frame (2-byte aligned dword)
             __builtin_memcpy(&d0, __builtin_assume_aligned(p0, 2), 4);
@@ -403,7 +412,7 @@ __device__ __forceinline__ void outlier_tile(const OutlierK& k, uint32_t bx, uin
 
         // ---- the lane's frame block; pads above N are set to MAXV
         uint32_t raw[C][NW];
-        load_block<B, C, F>(k.stack + pix * B, sh.off, t0, raw);
+        load_block<B, C, F>(k.stack + pix * B, sh.off + g * TS::OS, raw);
         if (N < (uint32_t)(G * F)) {  // grid-uniform: only stacks that do not fill the tile
 #pragma unroll
             for (int i = 0; i < NW; ++i) {
@@ -552,27 +561,27 @@ __device__ __forceinline__ void outlier_tile(const OutlierK& k, uint32_t bx, uin
         // ---- leave the lane's summary and the pixel's statistics in LDS
         {
             const int32_t nv = min((int32_t)N - (int32_t)t0, (int32_t)F);  // valid frames in this block
+            const uint32_t sl = TS::slot(g, p);
 #pragma unroll
             for (int w = 0; w < MW; ++w) {
                 const int32_t nw = nv - 32 * w;
                 const uint32_t vm = nw >= 32 ? 0xFFFFFFFFu : (nw <= 0 ? 0u : (1u << nw) - 1u);
-                sh.mask[lane * MW + w] = m[w] & vm;
+                sh.mask[w][sl] = m[w] & vm;
             }
-            sh.lmax[lane] = lmax;
-            sh.lmaxi[lane] = lmaxi;
+            sh.lmax[sl] = lmax;
+            sh.lmaxi[sl] = lmaxi;
         }
         if (g == 0) {
-            typename TS::Pix& rec = sh.px[p];
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
-                rec.med[c] = med[c];
-                rec.iqr_inv[c] = iqr_inv[c];
+                sh.med[c][p] = med[c];
+                sh.iqr_inv[c][p] = iqr_inv[c];
             }
 #pragma unroll
             for (int c = 0; c < C; ++c) {
-                rec.s0[c] = raw[c][0] & 0xFFFFu;
-                rec.sum_all[c] = sum_all[c];
-                rec.sum_out[c] = sum_out[c];
+                sh.s0[c][p] = raw[c][0] & 0xFFFFu;
+                sh.sum_all[c][p] = sum_all[c];
+                sh.sum_out[c][p] = sum_out[c];
             }
         }
     }
@@ -584,12 +593,11 @@ __device__ __forceinline__ void outlier_tile(const OutlierK& k, uint32_t bx, uin
     const uint32_t x = bx * P + p;
     if (x >= k.out_w) return;
     const uint64_t pix = ((uint64_t)y * k.stack_w + x) * C;
-    const typename TS::Pix& rec = sh.px[p];
     float med[4], iqr_inv[4], wi[4], sg[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-        med[c] = rec.med[c];
-        iqr_inv[c] = rec.iqr_inv[c];
+        med[c] = sh.med[c][p];
+        iqr_inv[c] = sh.iqr_inv[c][p];
         wi[c] = ABS ? k.w[c] : k.w[c] * iqr_inv[c];
         sg[c] = signum_f(k.w[c]);
     }
@@ -597,7 +605,7 @@ __device__ __forceinline__ void outlier_tile(const OutlierK& k, uint32_t bx, uin
 #pragma unroll
     for (int q = 0; q < G; ++q)
 #pragma unroll
-        for (int w = 0; w < MW; ++w) gm[q][w] = sh.mask[(p * G + q) * MW + w];
+        for (int w = 0; w < MW; ++w) gm[q][w] = sh.mask[w][TS::slot(q, p)];
 
     uint32_t num = 0, first_idx = NONE, last_idx = 0, first_non = NONE;
 #pragma unroll
@@ -621,8 +629,8 @@ __device__ __forceinline__ void outlier_tile(const OutlierK& k, uint32_t bx, uin
     if (k.outlier == 2 || k.stats != nullptr) {
 #pragma unroll
         for (int q = 0; q < G; ++q) {
-            const uint32_t li = sh.lmaxi[p * G + q];
-            const float lm = sh.lmax[p * G + q];
+            const uint32_t li = sh.lmaxi[TS::slot(q, p)];
+            const float lm = sh.lmax[TS::slot(q, p)];
             if (li != NONE && lm > max_d) { max_d = lm; max_index = li; }
         }
     }
@@ -631,7 +639,7 @@ __device__ __forceinline__ void outlier_tile(const OutlierK& k, uint32_t bx, uin
     constexpr uint32_t maxv = MAXV;
     auto sample = [&](uint32_t t, uint32_t (&s)[4]) {
 #pragma unroll
-        for (int ch = 0; ch < C; ++ch) s[ch] = load_sample<B>(k.stack + sh.off[t], pix + (uint64_t)ch);
+        for (int ch = 0; ch < C; ++ch) s[ch] = load_sample<B>(k.stack + sh.off[TS::oidx(t)], pix + (uint64_t)ch);
     };
     auto dist_of = [&](uint32_t t, uint32_t (&s)[4]) -> float {
         sample(t, s);
@@ -645,7 +653,7 @@ __device__ __forceinline__ void outlier_tile(const OutlierK& k, uint32_t bx, uin
     if (k.background == 2) {
         float mean[4];
 #pragma unroll
-        for (int c = 0; c < C; ++c) mean[c] = (float)rec.sum_all[c] / (float)N;
+        for (int c = 0; c < C; ++c) mean[c] = (float)sh.sum_all[c][p] / (float)N;
         if (num == 1) {
             sample(first_idx, s);
 #pragma unroll
@@ -654,7 +662,7 @@ __device__ __forceinline__ void outlier_tile(const OutlierK& k, uint32_t bx, uin
         } else if (num > 1) {
 #pragma unroll
             for (int c = 0; c < C; ++c)
-                pixel[c] = sat_cast(roundf(mean[c] * ((float)N / (float)(N - num)) - (float)rec.sum_out[c] / (float)N), maxv);
+                pixel[c] = sat_cast(roundf(mean[c] * ((float)N / (float)(N - num)) - (float)sh.sum_out[c][p] / (float)N), maxv);
         } else {
 #pragma unroll
             for (int c = 0; c < C; ++c) pixel[c] = sat_cast(roundf(mean[c]), maxv);
@@ -694,7 +702,7 @@ __device__ __forceinline__ void outlier_tile(const OutlierK& k, uint32_t bx, uin
         }
         if (bg_index == 0) {
 #pragma unroll
-            for (int c = 0; c < C; ++c) s[c] = rec.s0[c];
+            for (int c = 0; c < C; ++c) s[c] = sh.s0[c][p];
         } else {
             sample(bg_index, s);
         }
@@ -764,7 +772,7 @@ __device__ __forceinline__ void outlier_tile(const OutlierK& k, uint32_t bx, uin
                     }
                 }
 #pragma unroll
-                for (int c = 0; c < C; ++c) s[c] = sat_cast(roundf((float)rec.sum_out[c] / (float)num), maxv);
+                for (int c = 0; c < C; ++c) s[c] = sat_cast(roundf((float)sh.sum_out[c][p] / (float)num), maxv);
                 dist = mean_dist / (float)num;
             } else {
                 sel = k.outlier == 0 ? first_idx : (k.outlier == 1 ? last_idx : max_index);
@@ -806,10 +814,16 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t nb) {
     return xcd * q + min(xcd, r) + idx;
 }
 
+// Waves per SIMD the register allocator is asked to fit without spilling: 3-channel 32-frame
+// tiles fit five (absolute thresholds) or four (relative thresholds: three rank searches),
+// 4-channel tiles one fewer; the 64-frame tiles get the whole register file.
+constexpr int waves_hint(int F, int C, bool absolute) { return F > 32 ? 1 : (absolute ? 5 : 4) - (C > 3 ? 1 : 0); }
+
 template <int B, int C, int G, int F, bool ABS>
-__global__ void __launch_bounds__(256) outlier_fast_kernel(OutlierK k, uint32_t tiles_x) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(waves_hint(F, C, ABS))))
+outlier_fast_kernel(OutlierK k, uint32_t tiles_x) {
     __shared__ TileShared<C, G, F> sh;
-    load_offsets<B>(sh.off, G * F, k.pos_off, k.n);
+    load_offsets<B, F>(sh.off, G * F, k.pos_off, k.n);
     __syncthreads();
     const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
     outlier_tile<B, C, G, F, ABS>(k, tile % tiles_x, tile / tiles_x, sh);

@@ -1,11 +1,11 @@
-// kernels_batch_b1_c4.hip -- persistent batched outlier kernel, 8-bit samples, 4 channels.
+// kernels_batch_b1_c4.hip -- batched outlier kernels, 8-bit samples, 4 channels.
 #include "k_batch.h"
 
 namespace cp {
 
-int launch_batch_b1_c4(const BatchJobK* jobs, uint32_t njobs, uint32_t total, unsigned int* counter, int shape,
-                       bool absolute, int cus, hipStream_t s) {
-    return launch_batch_bc<1, 4>(jobs, njobs, total, counter, shape, absolute, cus, s);
+int launch_batch_b1_c4(const BatchJobK* jobs, uint32_t njobs, uint32_t total, int shape, bool absolute,
+                       hipStream_t s) {
+    return launch_batch_bc<1, 4>(jobs, njobs, total, shape, absolute, s);
 }
 
 }  // namespace cp

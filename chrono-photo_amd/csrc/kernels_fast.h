@@ -20,7 +20,7 @@ constexpr TileShape kShapes[kNumShapes] = {{1, 16}, {1, 32}, {2, 16}, {2, 32}, {
 constexpr int kNumBuckets = 6;
 constexpr uint32_t kBucketMax[kNumBuckets] = {16, 32, 64, 128, 256, 512};
 constexpr int kBucketCand[kNumBuckets][2] = {{0, -1}, {1, 2}, {3, 4}, {5, 6}, {7, 8}, {9, -1}};
-// Shapes the persistent batched kernel carries (F <= 32 keeps its register budget).
+// Shapes the batched kernels carry (F <= 32 keeps their register budget).
 constexpr int kBatchShapeMax = 7;
 
 inline int bucket_for(uint32_t n) {

@@ -188,8 +188,8 @@ CP_API int cp_video_windows(const cp_frame_range* video_in, const cp_frame_range
                             int32_t* win_frame, int32_t cap, int32_t* n_out);
 
 /* ------------------------------------------------------------------- batching
- * Many composites in one persistent launch (no reference counterpart: the reference runs
- * one processor per output, src/main.rs:260-330).  Outputs are device pointers. */
+ * Many composites in a few launches (no reference counterpart: the reference runs one
+ * processor per output, src/main.rs:260-330).  Outputs are device pointers. */
 typedef struct {
     int32_t mode;               /* CP_MODE_* */
     const cp_stack* stack;
@@ -203,10 +203,11 @@ typedef struct {
 typedef struct cp_batch cp_batch;
 
 /* Validate a list of jobs, stage all their per-call tables in device memory owned by the
- * batch, and capture the launch sequence (one persistent kernel per sample width and
- * channel count) in a HIP graph.  Outlier jobs with up to 256 frames, 3-4 channels and no
- * sample subset run in the persistent kernel; every other job runs as its own launch.  The
- * caller's arrays are copied; output pointers must stay valid while the batch exists. */
+ * batch, and capture the launch sequence in a HIP graph: 8-bit outlier jobs with up to 256
+ * frames, 3-4 channels and no sample subset are grouped by tile shape and threshold kind,
+ * one launch per group covering all its jobs' tiles; every other job runs as its own
+ * launch.  The caller's arrays are copied; output pointers must stay valid while the batch
+ * exists. */
 CP_API int cp_batch_create(cp_ctx* ctx, const cp_job* jobs, uint32_t n_jobs, uint32_t flags,
                            cp_batch** out);
 /* Replay the batch on the context stream (asynchronous). */

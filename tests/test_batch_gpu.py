@@ -1,4 +1,4 @@
-"""Batched launches (one persistent kernel over many stacks, captured in a HIP graph) give
+"""Batched launches (one kernel per tile-shape group over many stacks, captured in a HIP graph) give
 byte-identical results to one launch per stack, and the autotuner's tile choice never
 changes results."""
 import numpy as np
