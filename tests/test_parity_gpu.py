@@ -29,7 +29,7 @@ def make_stack(seed, n, h, w, c, bits=8, outlier_frac=0.15):
 
 
 def run_both(ctx, x, thr_abs, lo, hi, bg, sel, weights=(1, 1, 1, 1), fade=None, slots=None,
-             crop=None, sample=None, seed=42, force_generic=False, stream=True):
+             crop=None, sample=None, seed=42, force_generic=False):
     bps = x.dtype.itemsize
     st = cpa.Stack.from_array(ctx, x)
     thr = cpa.Threshold(thr_abs, lo, hi)
@@ -39,7 +39,7 @@ def run_both(ctx, x, thr_abs, lo, hi, bg, sel, weights=(1, 1, 1, 1), fade=None, 
     if crop is not None:
         cr = crop
     img, mask, stats = proc.process(st, image_indices=slots, crop=cr, with_stats=True,
-                                    force_generic=force_generic, stream=stream)
+                                    force_generic=force_generic)
     ofade = O.Fade.none() if fade is None else O.Fade(fade[0], fade[1], [f for f, _ in fade[2]],
                                                         [v for _, v in fade[2]])
     oimg, omask, owarn, ost = O.outlier(
@@ -74,9 +74,9 @@ MODES = list(itertools.product([0, 1, 2, 3], [0, 1, 2, 3, 4, 5]))
 def test_outlier_all_modes_u8_rgb(ctx, bg, sel, absolute):
     x = make_stack(100 + bg * 7 + sel, 17, 13, 19, 3)
     lo, hi = (0.05, 0.2) if absolute else (2.0, 4.0)
-    for generic, stream in ((False, True), (False, False), (True, True)):
-        gpu, ref = run_both(ctx, x, absolute, lo, hi, bg, sel, force_generic=generic, stream=stream)
-        assert_same(gpu, ref, f"bg={bg} sel={sel} abs={absolute} generic={generic} stream={stream}")
+    for generic in (False, True):
+        gpu, ref = run_both(ctx, x, absolute, lo, hi, bg, sel, force_generic=generic)
+        assert_same(gpu, ref, f"bg={bg} sel={sel} abs={absolute} generic={generic}")
 
 
 @pytest.mark.parametrize("sel", [0, 2, 3, 4, 5])
@@ -84,9 +84,8 @@ def test_outlier_all_modes_u8_rgb(ctx, bg, sel, absolute):
 def test_outlier_u16_rgb(ctx, sel, absolute):
     x = make_stack(300 + sel, 21, 11, 14, 3, bits=16)
     lo, hi = (0.05, 0.2) if absolute else (2.0, 4.0)
-    for stream in (True, False):
-        gpu, ref = run_both(ctx, x, absolute, lo, hi, 0, sel, stream=stream)
-        assert_same(gpu, ref, f"u16 sel={sel} stream={stream}")
+    gpu, ref = run_both(ctx, x, absolute, lo, hi, 0, sel)
+    assert_same(gpu, ref, f"u16 sel={sel}")
 
 
 @pytest.mark.parametrize("c", [1, 2, 4])
@@ -100,19 +99,16 @@ def test_outlier_channel_counts_and_weights(ctx, c):
 
 
 @pytest.mark.parametrize("bits,c,n", [(16, 3, 200), (8, 3, 120), (8, 4, 60), (16, 4, 30)])
-def test_outlier_streaming_many_tiles(ctx, bits, c, n):
-    """Enough tiles that every persistent workgroup walks several of them (and prefetches),
-    a last tile per row that is partly past the image, and per-frame crop origins that put
-    every frame row at a different alignment."""
+def test_outlier_many_tiles_with_crops(ctx, bits, c, n):
+    """Many tiles per row, a last tile per row that is partly past the image, and per-frame
+    crop origins that put every frame row at a different alignment."""
     x = make_stack(1100 + bits + c + n, n, 40, 1003, c, bits=bits, outlier_frac=0.05)
     rng = np.random.default_rng(n)
     crop_xy = rng.integers(0, 7, (n, 2)).astype(np.int32)
     crop = (crop_xy, 996, 33)
     for absolute, lo, hi, bg, sel in ((True, 0.05, 0.2, 0, 4), (False, 2.0, 4.0, 2, 3)):
         gpu, ref = run_both(ctx, x, absolute, lo, hi, bg, sel, crop=crop)
-        assert_same(gpu, ref, f"stream bits={bits} c={c} n={n} abs={absolute}")
-        gpu2, _ = run_both(ctx, x, absolute, lo, hi, bg, sel, crop=crop, stream=False)
-        assert_same(gpu2, ref, f"direct bits={bits} c={c} n={n} abs={absolute}")
+        assert_same(gpu, ref, f"bits={bits} c={c} n={n} abs={absolute}")
 
 
 @pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 8, 31, 32, 33, 64, 65, 100])
