@@ -836,11 +836,11 @@ def batch_run(ctx: Context, jobs):
 
 
 def autotune(ctx: Context, bytes_per_sample: int, channels: int):
-    """Runs the tile-shape autotuner; returns [(lanes_per_pixel, frames_per_lane)] per bucket
-    (<=16, 32, 64, 128, 256, 512 frames)."""
+    """Runs the tile-shape autotuner; returns [(lanes_per_pixel, frames_per_lane,
+    pixels_per_lane)] per bucket (<=16, 32, 64, 128, 256, 512 frames)."""
     out = (C.c_int32 * 6)()
     _check(ctx.lib.cp_autotune(ctx.handle, bytes_per_sample, channels, out))
-    return [(v >> 8, v & 0xFF) for v in out]
+    return [((v >> 8) & 0xFF, v & 0xFF, v >> 16) for v in out]
 
 
 # -------------------------------------------------------------------- serving

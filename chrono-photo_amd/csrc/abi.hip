@@ -1,5 +1,6 @@
 // abi.hip -- C ABI implementation: runtime, stacks, option values, compositing entry
 // points.  The contract of each entry point is documented in include/chrono_hip.h.
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -198,7 +199,7 @@ int cp_ctx::shape_for(uint32_t bps, uint32_t channels, uint32_t n) const {
     const int b = cp::bucket_for(n);
     if (b < 0 || bps < 1 || bps > 2 || channels < 1 || channels > 4) return -1;
     const int t = tuned[bps - 1][channels - 1][b];
-    return t >= 0 ? t : cp::kBucketCand[b][0];
+    return t >= 0 ? t : cp::kBucketCand[bps - 1][b][0];
 }
 
 int cp_ctx::charge(uint64_t bytes) {
@@ -280,7 +281,7 @@ CP_API int cp_stack_create(cp_ctx* ctx, uint32_t width, uint32_t height, uint32_
     s->channels = channels;
     s->bps = bytes_per_sample;
     s->capacity = capacity;
-    uint64_t bytes = s->frame_bytes() * capacity;
+    uint64_t bytes = s->frame_bytes() * capacity + kStackSlack;
     int rc = ctx->charge(bytes);
     if (rc) { delete s; return rc; }
     (void)hipSetDevice(ctx->device);
@@ -289,6 +290,7 @@ CP_API int cp_stack_create(cp_ctx* ctx, uint32_t width, uint32_t height, uint32_
         delete s;
         return fail(CP_ERR_HIP, "hipMalloc of %llu stack bytes failed", (unsigned long long)bytes);
     }
+    s->alloc_bytes = bytes;
     s->owned = true;
     *out = s;
     return CP_OK;
@@ -312,6 +314,7 @@ CP_API int cp_stack_wrap(cp_ctx* ctx, void* device_ptr, uint32_t width, uint32_t
     s->bps = bytes_per_sample;
     s->capacity = s->count = n_frames;
     s->dev = (uint8_t*)device_ptr;
+    s->alloc_bytes = s->frame_bytes() * n_frames;
     s->owned = false;
     *out = s;
     return CP_OK;
@@ -375,7 +378,7 @@ CP_API int cp_stack_destroy(cp_stack* st) {
         (void)hipSetDevice(st->ctx->device);
         (void)hipStreamSynchronize(st->ctx->stream);
         (void)hipFree(st->dev);
-        st->ctx->uncharge(st->frame_bytes() * st->capacity);
+        st->ctx->uncharge(st->alloc_bytes);
     }
     delete st;
     return CP_OK;
@@ -486,7 +489,8 @@ CP_API int cp_outlier(cp_ctx* ctx, const cp_stack* st, const cp_frames* frames,
         k.warnings = (unsigned long long*)ctx->warn.ptr;
         CP_HIP_TRY(hipMemsetAsync(ctx->warn.ptr, 0, 8, s));
     }
-    if ((rc = launch_outlier(ctx, st, k, flags, s))) return rc;
+    const uint64_t max_pos = *std::max_element(prep.pos_off.begin(), prep.pos_off.end());
+    if ((rc = launch_outlier(ctx, st, k, max_pos, flags, s))) return rc;
     if (!dev_out) {
         CP_HIP_TRY(hipMemcpyAsync(out->px, k.out_px, px_bytes, hipMemcpyDeviceToHost, s));
         if (out->mask)

@@ -310,11 +310,11 @@ extern "C" CP_API int cp_autotune(cp_ctx* ctx, uint32_t bps, uint32_t channels, 
               hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess &&
               hipMemcpyAsync(d_pos, pos.data(), 8 * NMAX, hipMemcpyHostToDevice, s) == hipSuccess;
     for (int b = 0; ok && b < kNumBuckets; ++b) {
-        int best = kBucketCand[b][0];
-        if (kBucketCand[b][1] >= 0) {
+        int best = kBucketCand[bps - 1][b][0];
+        if (kBucketCand[bps - 1][b][1] >= 0) {
             float best_ms = 1e30f;
             for (int c = 0; c < 2; ++c) {
-                const int shape = kBucketCand[b][c];
+                const int shape = kBucketCand[bps - 1][b][c];
                 OutlierK k{};
                 k.stack = st->dev;
                 k.pos_off = d_pos;
@@ -340,7 +340,7 @@ extern "C" CP_API int cp_autotune(cp_ctx* ctx, uint32_t bps, uint32_t channels, 
             }
         }
         ctx->tuned[bps - 1][channels - 1][b] = (int8_t)best;
-        if (shapes) shapes[b] = (kShapes[best].G << 8) | kShapes[best].F;
+        if (shapes) shapes[b] = (kShapes[best].X << 16) | (kShapes[best].G << 8) | kShapes[best].F;
     }
     (void)hipStreamSynchronize(s);
     if (d_pos) (void)hipFree(d_pos);

@@ -37,7 +37,7 @@ outlier_batch_kernel(const BatchJobK* __restrict__ jobs, uint32_t njobs) {
     load_offsets<B, F>(sh.off, G * F, k.pos_off, k.n);
     __syncthreads();
     const uint32_t t = tile - J.tile0;
-    outlier_tile<B, C, G, F, ABS>(k, t % J.tiles_x, t / J.tiles_x, sh);
+    outlier_tile<B, C, G, F, 1, ABS>(k, t % J.tiles_x, t / J.tiles_x, sh);
 }
 
 template <int B, int C, int SH, bool ABS>

@@ -299,26 +299,31 @@ __device__ __forceinline__ float frame_dist(const uint32_t (&s)[4], const float 
 // Layouts keep every access of a wave in distinct LDS banks: the offset table leaves two
 // spare entries after each lane block, so the G lanes of a pixel (which read entries
 // g*F + j at once) start in different banks; per-lane summaries sit at slot(g, p), which
-// rotates each frame block's row by 4 so that the finishing lanes (fixed g, consecutive p)
-// and the writing lanes (8 pixels x G blocks) both spread over the banks; per-pixel values
-// are stored channel-major.
-template <int C, int G, int F>
+// rotates each frame block's row (by 4g for one pixel per lane; for two pixels per lane by
+// an odd/even pair of offsets that spreads the even and odd pixels) so that the finishing
+// lanes (fixed g, consecutive p) and the writing lanes (the pixels of a wave x G blocks)
+// both spread over the banks; per-pixel values are stored channel-major.
+template <int C, int G, int F, int X = 1>
 struct TileShared {
-    static constexpr int P = 256 / G;
-    static constexpr int MW = F > 32 ? 2 : 1;  // mask words per lane
+    static constexpr int P = X * 256 / G;      // pixels per tile
+    static constexpr int NS = G * P;           // lane summary slots (X per lane)
+    static constexpr int MW = F > 32 ? 2 : 1;  // mask words per lane and pixel
     static constexpr int OS = F + 2;           // offset-table stride of a lane block
     uint64_t off[G * OS];      // frame byte offsets at oidx(t); positions >= N repeat N-1
-    uint32_t mask[MW][256];    // [w][slot]: outlier bits of frames g*F + 32w + b
-    float lmax[256];           // [slot]: the lane's largest outlier distance (valid when lmaxi != ~0)
-    uint32_t lmaxi[256];       // [slot]: its first frame
+    uint32_t mask[MW][NS];     // [w][slot]: outlier bits of frames g*F + 32w + b
+    float lmax[NS];            // [slot]: the largest outlier distance (valid when lmaxi != ~0)
+    uint32_t lmaxi[NS];        // [slot]: its first frame
     float med[4][P];
     float iqr_inv[4][P];
     uint32_t s0[4][P];       // frame 0 samples
     uint32_t sum_all[4][P];  // average modes
     uint32_t sum_out[4][P];
     __device__ static __forceinline__ uint32_t oidx(uint32_t t) { return t + 2u * (t / (uint32_t)F); }
+    __device__ static __forceinline__ uint32_t rot(uint32_t g) {
+        return X == 1 ? 4u * g : (g & 1u) + (g >> 1) * (64u / (uint32_t)G);
+    }
     __device__ static __forceinline__ uint32_t slot(uint32_t g, uint32_t p) {
-        return g * (uint32_t)P + ((p + 4u * g) & (uint32_t)(P - 1));
+        return g * (uint32_t)P + ((p + rot(g)) & (uint32_t)(P - 1));
     }
 };
 
@@ -331,16 +336,48 @@ __device__ __forceinline__ void load_offsets(uint64_t* off, uint32_t count, cons
     for (uint32_t i = threadIdx.x; i < count; i += 256) off[i + 2u * (i / (uint32_t)F)] = pos_off[min(i, n - 1)] * B;
 }
 
-// Loads the F frames of one lane block into raw[c][j/2] (frame j in half j&1); base is the
-// pixel's address in stack slot 0 and off the block's F frame byte offsets.
-template <int B, int C, int F>
+// The 12 bytes at p (2-byte aligned) as three dwords: one 16-byte load from the dword
+// boundary at or below p, realigned by the byte shift (0 or 2).  Reads at most 4 bytes
+// past p + 12.
+__device__ __forceinline__ void load12_a2(const uint8_t* p, uint32_t (&d)[3]) {
+    const uint32_t s = (uint32_t)(uintptr_t)p & 3u;
+    uint32_t q[4];
+    __builtin_memcpy(q, __builtin_assume_aligned(p - s, 4), 16);
+    d[0] = __builtin_amdgcn_alignbyte(q[1], q[0], s);
+    d[1] = __builtin_amdgcn_alignbyte(q[2], q[1], s);
+    d[2] = __builtin_amdgcn_alignbyte(q[3], q[2], s);
+}
+
+// Loads the F frames of one lane block into raw[e*C + c][j/2] (pixel e of the lane, frame j
+// in half j&1); base is the lane's first pixel in stack slot 0 and off the block's F frame
+// byte offsets.  Two pixels per lane (16-bit stacks) take one 12- or 16-byte load per frame.
+template <int B, int C, int F, int X>
 __device__ __forceinline__ void load_block(const uint8_t* __restrict__ base, const uint64_t* off,
-                                           uint32_t (&raw)[C][F / 2]) {
+                                           uint32_t (&raw)[X * C][F / 2]) {
+    static_assert(X == 1 || (X == 2 && B == 2 && C >= 3), "two pixels per lane: 16-bit, 3 or 4 channels");
 #pragma unroll
     for (int i = 0; i < F / 2; ++i) {
         const uint8_t* p0 = base + off[2 * i];
         const uint8_t* p1 = base + off[2 * i + 1];
-        if constexpr (B == 2 && C == 3) {
+        if constexpr (X == 2 && C == 3) {
+            uint32_t a[3], b[3];  // [c0|c1], [c2|c0'], [c1'|c2'] of pixels 0 and 1
+            load12_a2(p0, a);
+            load12_a2(p1, b);
+#pragma unroll
+            for (int h = 0; h < 3; ++h) {
+                raw[2 * h][i] = __builtin_amdgcn_perm(b[h], a[h], 0x05040100u);
+                raw[2 * h + 1][i] = __builtin_amdgcn_perm(b[h], a[h], 0x07060302u);
+            }
+        } else if constexpr (X == 2 && C == 4) {
+            uint32_t a[4], b[4];
+            __builtin_memcpy(a, __builtin_assume_aligned(p0, 8), 16);
+            __builtin_memcpy(b, __builtin_assume_aligned(p1, 8), 16);
+#pragma unroll
+            for (int h = 0; h < 4; ++h) {
+                raw[2 * h][i] = __builtin_amdgcn_perm(b[h], a[h], 0x05040100u);
+                raw[2 * h + 1][i] = __builtin_amdgcn_perm(b[h], a[h], 0x07060302u);
+            }
+        } else if constexpr (B == 2 && C == 3) {
             uint32_t d0, d1;  // channels 0-1 of each frame (2-byte aligned dword)
             __builtin_memcpy(&d0, __builtin_assume_aligned(p0, 2), 4);
             __builtin_memcpy(&d1, __builtin_assume_aligned(p1, 2), 4);
@@ -385,211 +422,29 @@ __device__ __forceinline__ void load_block(const uint8_t* __restrict__ base, con
     }
 }
 
-// One tile: P = 256/G pixels of row y starting at pixel bx * P.  sh.off must hold the
-// tile's frame offsets (load_offsets) and be visible to all lanes.  Every lane of the
-// workgroup must call this (it synchronises once).
-template <int B, int C, int G, int F, bool ABS>
-__device__ __forceinline__ void outlier_tile(const OutlierK& k, uint32_t bx, uint32_t y,
-                                             TileShared<C, G, F>& sh) {
-    static_assert(F % 4 == 0 && F <= 64 && G <= 8, "tile shape");
-    using TS = TileShared<C, G, F>;
+// fn(std::integral_constant<int, I>) for I = 0 .. N-1.
+template <int I, int N>
+struct StaticFor {
+    template <typename Fn>
+    __device__ static __forceinline__ void run(Fn&& fn) {
+        if constexpr (I < N) {
+            fn(std::integral_constant<int, I>{});
+            StaticFor<I + 1, N>::run(fn);
+        }
+    }
+};
+
+// Finishes pixel p of a tile from the lanes' summaries in LDS: background and outlier
+// selection / blending (src/chrono.rs:290-493), output and statistics.
+template <int B, int C, int G, int F, int X, bool ABS>
+__device__ __forceinline__ void finish_pixel(const OutlierK& k, uint32_t bx, uint32_t y, uint32_t p,
+                                             const TileShared<C, G, F, X>& sh) {
+    using TS = TileShared<C, G, F, X>;
     constexpr int P = TS::P;
     constexpr int MW = TS::MW;
-    constexpr int NW = F / 2;
     constexpr uint32_t MAXV = B == 1 ? 255u : 65535u;
     constexpr uint32_t NONE = 0xFFFFFFFFu;
     const uint32_t N = k.n;
-
-    const uint32_t lane = threadIdx.x;
-    const uint32_t g = lane % G;
-    const uint32_t t0 = g * F;
-    const int32_t ntot = G * F;
-    {
-        const uint32_t p = lane / G;
-        const uint32_t x = bx * P + p;
-        const bool px_ok = x < k.out_w;
-        const uint64_t pix = ((uint64_t)y * k.stack_w + (px_ok ? x : 0u)) * C;
-
-        // ---- the lane's frame block; pads above N are set to MAXV
-        uint32_t raw[C][NW];
-        load_block<B, C, F>(k.stack + pix * B, sh.off + g * TS::OS, raw);
-        if (N < (uint32_t)(G * F)) {  // grid-uniform: only stacks that do not fill the tile
-#pragma unroll
-            for (int i = 0; i < NW; ++i) {
-                const uint32_t t = t0 + 2 * i;
-#pragma unroll
-                for (int c = 0; c < C; ++c) {
-                    uint32_t r = raw[c][i];
-                    r = t + 1 >= N ? (r & 0xFFFFu) | (MAXV << 16) : r;
-                    r = t >= N ? MAXV | (MAXV << 16) : r;
-                    raw[c][i] = r;
-                }
-            }
-        }
-
-        // ---- medians / inverse IQR per channel
-        const RankPlan plan = rank_plan_core(N, ABS);
-        float med[4] = {0, 0, 0, 0}, iqr_inv[4] = {0, 0, 0, 0};
-        {
-            uint32_t m0[C], m1[C], a0[C], a1[C], b0[C], b1[C];
-            rank_pairs<B, G, NW, C, ABS>(raw, plan.m0, plan.m1, ntot, m0, m1);
-            if constexpr (!ABS) {
-                rank_pairs<B, G, NW, C, false>(raw, plan.a0, plan.a1, ntot, a0, a1);
-                rank_pairs<B, G, NW, C, false>(raw, plan.b0, plan.b1, ntot, b0, b1);
-            }
-#pragma unroll
-            for (int c = 0; c < C; ++c) {
-                float mc, ic;
-                if constexpr (ABS) stats_from_values(plan, m0[c], m1[c], 0, 0, 0, 0, true, &mc, &ic);
-                else stats_from_values(plan, m0[c], m1[c], a0[c], a1[c], b0[c], b1[c], false, &mc, &ic);
-                // channels without weight keep zero statistics
-                med[c] = k.w[c] == 0.0f ? 0.0f : mc;
-                iqr_inv[c] = k.w[c] == 0.0f ? 0.0f : ic;
-            }
-        }
-
-        // ---- classification over the lane's frames
-        float wi[4], sg[4];
-        bool unit = ABS, pos = true;
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            wi[c] = ABS ? k.w[c] : k.w[c] * iqr_inv[c];
-            sg[c] = signum_f(k.w[c]);
-            if (c < C) {
-                unit = unit && k.w[c] == 1.0f;
-                pos = pos && !(sg[c] < 0.0f);
-            }
-        }
-        const float tsq = k.th.min * k.th.min;
-        const bool want_max = k.outlier == 2 || k.stats != nullptr;
-        const bool want_sums = k.background == 2 || k.outlier == 3;
-        const bool full_tile = N == (uint32_t)(G * F);
-        uint32_t m[MW];
-#pragma unroll
-        for (int w = 0; w < MW; ++w) m[w] = 0;
-        float lmax = tsq;
-        uint32_t lmaxi = NONE;
-        uint32_t sum_all[4] = {0, 0, 0, 0}, sum_out[4] = {0, 0, 0, 0};
-
-        // Bound check: with non-negative weights every frame's f32 distance is at most the
-        // same f32 expression on the per-channel maximum deviation from the median (correct
-        // rounding is monotone), so when that bound is below the threshold no frame of this
-        // lane can be an outlier and the per-frame pass is skipped.
-        bool may_outlier = true;
-        if (pos && full_tile) {
-            float bound = 0.0f;
-#pragma unroll
-            for (int c = 0; c < C; ++c) {
-                us2 mn = {0xFFFF, 0xFFFF}, mx = {0, 0};
-#pragma unroll
-                for (int i = 0; i < NW; ++i) {
-                    mn = __builtin_elementwise_min(mn, as_us2(raw[c][i]));
-                    mx = __builtin_elementwise_max(mx, as_us2(raw[c][i]));
-                }
-                const float lo = (float)min((uint32_t)mn.x, (uint32_t)mn.y);
-                const float hi = (float)max((uint32_t)mx.x, (uint32_t)mx.y);
-                const float dev = fmaxf(med[c] - lo, hi - med[c]);
-                const float a = unit ? dev : wi[c] * dev;
-                bound = c == 0 ? a * a : bound + a * a;
-            }
-            may_outlier = bound >= tsq;
-        }
-
-        // Frames run from last to first so that the mask shifts in bit j for frame j; the
-        // largest distance keeps the lowest frame among ties (>=).  lmax starts at the
-        // threshold, so `d >= lmax` also implies that the frame is an outlier.
-        auto classify = [&](auto unit_t, auto pos_t, auto max_t, auto sums_t, auto check_t) {
-            constexpr bool UNIT = decltype(unit_t)::value, POS = decltype(pos_t)::value;
-            constexpr bool MAXT = decltype(max_t)::value, SUMS = decltype(sums_t)::value;
-            constexpr bool CHECK = decltype(check_t)::value;
-#pragma unroll
-            for (int j = F - 1; j >= 0; --j) {
-                uint32_t xs[4];
-                float d = 0.0f;
-#pragma unroll
-                for (int c = 0; c < C; ++c) {
-                    const uint32_t w32 = raw[c][j >> 1];
-                    xs[c] = (j & 1) ? (w32 >> 16) : (w32 & 0xFFFFu);
-                    const float diff = med[c] - (float)xs[c];
-                    const float a = UNIT ? diff : wi[c] * diff;
-                    float term = a * a;
-                    if constexpr (!POS) term = sg[c] * term;
-                    d = c == 0 ? term : d + term;  // 0 + term0 == term0 up to the sign of zero
-                }
-                const bool o = d >= tsq;  // pads are masked out after the loop
-                m[j >> 5] = (m[j >> 5] << 1) | (o ? 1u : 0u);
-                bool in = true;
-                if constexpr (CHECK) in = t0 + (uint32_t)j < N;
-                if constexpr (MAXT) {
-                    if (in && d >= lmax) { lmax = d; lmaxi = t0 + (uint32_t)j; }
-                }
-                if constexpr (SUMS) {
-#pragma unroll
-                    for (int c = 0; c < C; ++c) sum_out[c] += (o && in) ? xs[c] : 0u;
-                }
-            }
-        };
-        using T_ = std::true_type;
-        using F_ = std::false_type;
-        if (may_outlier) {
-            if (!full_tile) classify(F_{}, F_{}, T_{}, T_{}, T_{});
-            else if (want_max) {
-                if (want_sums) classify(F_{}, F_{}, T_{}, T_{}, F_{});
-                else if (unit) classify(T_{}, T_{}, T_{}, F_{}, F_{});
-                else if (pos) classify(F_{}, T_{}, T_{}, F_{}, F_{});
-                else classify(F_{}, F_{}, T_{}, F_{}, F_{});
-            } else {
-                if (want_sums) classify(F_{}, F_{}, F_{}, T_{}, F_{});
-                else if (unit) classify(T_{}, T_{}, F_{}, F_{}, F_{});
-                else if (pos) classify(F_{}, T_{}, F_{}, F_{}, F_{});
-                else classify(F_{}, F_{}, F_{}, F_{}, F_{});
-            }
-        }
-        if (want_sums) {
-            // all-frame sums through v_sad_u16 against zero; pads (MAXV) are taken back out
-            const uint32_t npad = t0 + F > N ? min(t0 + F - N, (uint32_t)F) : 0u;
-#pragma unroll
-            for (int c = 0; c < C; ++c) {
-                uint32_t acc = 0;
-#pragma unroll
-                for (int i = 0; i < NW; ++i) acc = __builtin_amdgcn_sad_u16(raw[c][i], 0u, acc);
-                sum_all[c] = (uint32_t)grp_sum<G>((int32_t)(acc - npad * MAXV));
-                sum_out[c] = (uint32_t)grp_sum<G>((int32_t)sum_out[c]);
-            }
-        }
-
-        // ---- leave the lane's summary and the pixel's statistics in LDS
-        {
-            const int32_t nv = min((int32_t)N - (int32_t)t0, (int32_t)F);  // valid frames in this block
-            const uint32_t sl = TS::slot(g, p);
-#pragma unroll
-            for (int w = 0; w < MW; ++w) {
-                const int32_t nw = nv - 32 * w;
-                const uint32_t vm = nw >= 32 ? 0xFFFFFFFFu : (nw <= 0 ? 0u : (1u << nw) - 1u);
-                sh.mask[w][sl] = m[w] & vm;
-            }
-            sh.lmax[sl] = lmax;
-            sh.lmaxi[sl] = lmaxi;
-        }
-        if (g == 0) {
-#pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                sh.med[c][p] = med[c];
-                sh.iqr_inv[c][p] = iqr_inv[c];
-            }
-#pragma unroll
-            for (int c = 0; c < C; ++c) {
-                sh.s0[c][p] = raw[c][0] & 0xFFFFu;
-                sh.sum_all[c][p] = sum_all[c];
-                sh.sum_out[c][p] = sum_out[c];
-            }
-        }
-    }
-    __syncthreads();
-    if (lane >= (uint32_t)P) return;
-
-    // ---- finish one pixel per lane: background + outlier selection (src/chrono.rs:290-493)
-    const uint32_t p = lane;
     const uint32_t x = bx * P + p;
     if (x >= k.out_w) return;
     const uint64_t pix = ((uint64_t)y * k.stack_w + x) * C;
@@ -806,6 +661,239 @@ __device__ __forceinline__ void outlier_tile(const OutlierK& k, uint32_t bx, uin
     }
 }
 
+// One tile: P = X*256/G pixels of row y starting at pixel bx * P; lane (pl, g) holds frame
+// block g of pixels bx*P + pl*X + e, e < X.  sh.off must hold the tile's frame offsets
+// (load_offsets) and be visible to all lanes.  Every lane of the workgroup must call this
+// (it synchronises once).  With X = 2 the lanes read up to 16 bytes past their pixels'
+// samples (load12_a2, and pixel out_w of a row when out_w is odd); the launcher checks that
+// those bytes lie inside the stack allocation.
+template <int B, int C, int G, int F, int X, bool ABS>
+__device__ __forceinline__ void outlier_tile(const OutlierK& k, uint32_t bx, uint32_t y,
+                                             TileShared<C, G, F, X>& sh) {
+    static_assert(F % 4 == 0 && F <= 64 && G <= 8 && (X == 1 || X == 2), "tile shape");
+    using TS = TileShared<C, G, F, X>;
+    constexpr int P = TS::P;
+    constexpr int MW = TS::MW;
+    constexpr int NW = F / 2;
+    constexpr uint32_t MAXV = B == 1 ? 255u : 65535u;
+    constexpr uint32_t NONE = 0xFFFFFFFFu;
+    const uint32_t N = k.n;
+
+    const uint32_t lane = threadIdx.x;
+    const uint32_t g = lane % G;
+    const uint32_t t0 = g * F;
+    const int32_t ntot = G * F;
+    {
+        const uint32_t pl = lane / G;
+        const uint32_t x0 = bx * P + pl * X;
+        const uint64_t pix = ((uint64_t)y * k.stack_w + (x0 < k.out_w ? x0 : 0u)) * C;
+
+        // ---- the lane's frame block; pads above N are set to MAXV
+        uint32_t raw[X * C][NW];
+        load_block<B, C, F, X>(k.stack + pix * B, sh.off + g * TS::OS, raw);
+        if (N < (uint32_t)(G * F)) {  // grid-uniform: only stacks that do not fill the tile
+#pragma unroll
+            for (int i = 0; i < NW; ++i) {
+                const uint32_t t = t0 + 2 * i;
+#pragma unroll
+                for (int c = 0; c < X * C; ++c) {
+                    uint32_t r = raw[c][i];
+                    r = t + 1 >= N ? (r & 0xFFFFu) | (MAXV << 16) : r;
+                    r = t >= N ? MAXV | (MAXV << 16) : r;
+                    raw[c][i] = r;
+                }
+            }
+        }
+
+        // ---- medians / inverse IQR per pixel and channel
+        const RankPlan plan = rank_plan_core(N, ABS);
+        float med[X][4], iqr_inv[X][4];
+        StaticFor<0, X>::run([&](auto e_t) {
+            constexpr int E = decltype(e_t)::value;
+            const uint32_t (&re)[C][NW] = *reinterpret_cast<const uint32_t(*)[C][NW]>(&raw[E * C][0]);
+            uint32_t m0[C], m1[C], a0[C], a1[C], b0[C], b1[C];
+            rank_pairs<B, G, NW, C, ABS>(re, plan.m0, plan.m1, ntot, m0, m1);
+            if constexpr (!ABS) {
+                rank_pairs<B, G, NW, C, false>(re, plan.a0, plan.a1, ntot, a0, a1);
+                rank_pairs<B, G, NW, C, false>(re, plan.b0, plan.b1, ntot, b0, b1);
+            }
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                med[E][c] = 0.0f;
+                iqr_inv[E][c] = 0.0f;
+            }
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                float mc, ic;
+                if constexpr (ABS) stats_from_values(plan, m0[c], m1[c], 0, 0, 0, 0, true, &mc, &ic);
+                else stats_from_values(plan, m0[c], m1[c], a0[c], a1[c], b0[c], b1[c], false, &mc, &ic);
+                // channels without weight keep zero statistics
+                med[E][c] = k.w[c] == 0.0f ? 0.0f : mc;
+                iqr_inv[E][c] = k.w[c] == 0.0f ? 0.0f : ic;
+            }
+        });
+
+        // ---- classification over the lane's frames
+        float wi[X][4], sg[4];
+        bool unit = ABS, pos = true;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            sg[c] = signum_f(k.w[c]);
+#pragma unroll
+            for (int e = 0; e < X; ++e) wi[e][c] = ABS ? k.w[c] : k.w[c] * iqr_inv[e][c];
+            if (c < C) {
+                unit = unit && k.w[c] == 1.0f;
+                pos = pos && !(sg[c] < 0.0f);
+            }
+        }
+        const float tsq = k.th.min * k.th.min;
+        const bool want_max = k.outlier == 2 || k.stats != nullptr;
+        const bool want_sums = k.background == 2 || k.outlier == 3;
+        const bool full_tile = N == (uint32_t)(G * F);
+        uint32_t m[X][MW];
+        float lmax[X];
+        uint32_t lmaxi[X];
+        uint32_t sum_all[X][4], sum_out[X][4];
+#pragma unroll
+        for (int e = 0; e < X; ++e) {
+#pragma unroll
+            for (int w = 0; w < MW; ++w) m[e][w] = 0;
+            lmax[e] = tsq;
+            lmaxi[e] = NONE;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) sum_all[e][c] = sum_out[e][c] = 0;
+        }
+
+        // Frames run from last to first so that the mask shifts in bit j for frame j; the
+        // largest distance keeps the lowest frame among ties (>=).  lmax starts at the
+        // threshold, so `d >= lmax` also implies that the frame is an outlier.
+        auto classify = [&](auto e_t, auto unit_t, auto pos_t, auto max_t, auto sums_t, auto check_t) {
+            constexpr int E = decltype(e_t)::value;
+            constexpr bool UNIT = decltype(unit_t)::value, POS = decltype(pos_t)::value;
+            constexpr bool MAXT = decltype(max_t)::value, SUMS = decltype(sums_t)::value;
+            constexpr bool CHECK = decltype(check_t)::value;
+#pragma unroll
+            for (int j = F - 1; j >= 0; --j) {
+                uint32_t xs[4];
+                float d = 0.0f;
+#pragma unroll
+                for (int c = 0; c < C; ++c) {
+                    const uint32_t w32 = raw[E * C + c][j >> 1];
+                    xs[c] = (j & 1) ? (w32 >> 16) : (w32 & 0xFFFFu);
+                    const float diff = med[E][c] - (float)xs[c];
+                    const float a = UNIT ? diff : wi[E][c] * diff;
+                    float term = a * a;
+                    if constexpr (!POS) term = sg[c] * term;
+                    d = c == 0 ? term : d + term;  // 0 + term0 == term0 up to the sign of zero
+                }
+                const bool o = d >= tsq;  // pads are masked out after the loop
+                m[E][j >> 5] = (m[E][j >> 5] << 1) | (o ? 1u : 0u);
+                bool in = true;
+                if constexpr (CHECK) in = t0 + (uint32_t)j < N;
+                if constexpr (MAXT) {
+                    if (in && d >= lmax[E]) { lmax[E] = d; lmaxi[E] = t0 + (uint32_t)j; }
+                }
+                if constexpr (SUMS) {
+#pragma unroll
+                    for (int c = 0; c < C; ++c) sum_out[E][c] += (o && in) ? xs[c] : 0u;
+                }
+            }
+        };
+        using T_ = std::true_type;
+        using F_ = std::false_type;
+        StaticFor<0, X>::run([&](auto e_t) {
+            constexpr int E = decltype(e_t)::value;
+            // Bound check: with non-negative weights every frame's f32 distance is at most
+            // the same f32 expression on the per-channel maximum deviation from the median
+            // (correct rounding is monotone), so when that bound is below the threshold no
+            // frame of this lane can be an outlier and the per-frame pass is skipped.
+            bool may_outlier = true;
+            if (pos && full_tile) {
+                float bound = 0.0f;
+#pragma unroll
+                for (int c = 0; c < C; ++c) {
+                    us2 mn = {0xFFFF, 0xFFFF}, mx = {0, 0};
+#pragma unroll
+                    for (int i = 0; i < NW; ++i) {
+                        mn = __builtin_elementwise_min(mn, as_us2(raw[E * C + c][i]));
+                        mx = __builtin_elementwise_max(mx, as_us2(raw[E * C + c][i]));
+                    }
+                    const float lo = (float)min((uint32_t)mn.x, (uint32_t)mn.y);
+                    const float hi = (float)max((uint32_t)mx.x, (uint32_t)mx.y);
+                    const float dev = fmaxf(med[E][c] - lo, hi - med[E][c]);
+                    const float a = unit ? dev : wi[E][c] * dev;
+                    bound = c == 0 ? a * a : bound + a * a;
+                }
+                may_outlier = bound >= tsq;
+            }
+            if (may_outlier) {
+                if (!full_tile) classify(e_t, F_{}, F_{}, T_{}, T_{}, T_{});
+                else if (want_max) {
+                    if (want_sums) classify(e_t, F_{}, F_{}, T_{}, T_{}, F_{});
+                    else if (unit) classify(e_t, T_{}, T_{}, T_{}, F_{}, F_{});
+                    else if (pos) classify(e_t, F_{}, T_{}, T_{}, F_{}, F_{});
+                    else classify(e_t, F_{}, F_{}, T_{}, F_{}, F_{});
+                } else {
+                    if (want_sums) classify(e_t, F_{}, F_{}, F_{}, T_{}, F_{});
+                    else if (unit) classify(e_t, T_{}, T_{}, F_{}, F_{}, F_{});
+                    else if (pos) classify(e_t, F_{}, T_{}, F_{}, F_{}, F_{});
+                    else classify(e_t, F_{}, F_{}, F_{}, F_{}, F_{});
+                }
+            }
+        });
+        if (want_sums) {
+            // all-frame sums through v_sad_u16 against zero; pads (MAXV) are taken back out
+            const uint32_t npad = t0 + F > N ? min(t0 + F - N, (uint32_t)F) : 0u;
+#pragma unroll
+            for (int e = 0; e < X; ++e)
+#pragma unroll
+                for (int c = 0; c < C; ++c) {
+                    uint32_t acc = 0;
+#pragma unroll
+                    for (int i = 0; i < NW; ++i) acc = __builtin_amdgcn_sad_u16(raw[e * C + c][i], 0u, acc);
+                    sum_all[e][c] = (uint32_t)grp_sum<G>((int32_t)(acc - npad * MAXV));
+                    sum_out[e][c] = (uint32_t)grp_sum<G>((int32_t)sum_out[e][c]);
+                }
+        }
+
+        // ---- leave the lane's summaries and the pixels' statistics in LDS
+        {
+            const int32_t nv = min((int32_t)N - (int32_t)t0, (int32_t)F);  // valid frames in this block
+#pragma unroll
+            for (int e = 0; e < X; ++e) {
+                const uint32_t sl = TS::slot(g, pl * X + e);
+#pragma unroll
+                for (int w = 0; w < MW; ++w) {
+                    const int32_t nw = nv - 32 * w;
+                    const uint32_t vm = nw >= 32 ? 0xFFFFFFFFu : (nw <= 0 ? 0u : (1u << nw) - 1u);
+                    sh.mask[w][sl] = m[e][w] & vm;
+                }
+                sh.lmax[sl] = lmax[e];
+                sh.lmaxi[sl] = lmaxi[e];
+            }
+        }
+        if (g == 0) {
+#pragma unroll
+            for (int e = 0; e < X; ++e) {
+                const uint32_t p = pl * X + e;
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    sh.med[c][p] = med[e][c];
+                    sh.iqr_inv[c][p] = iqr_inv[e][c];
+                }
+#pragma unroll
+                for (int c = 0; c < C; ++c) {
+                    sh.s0[c][p] = raw[e * C + c][0] & 0xFFFFu;
+                    sh.sum_all[c][p] = sum_all[e][c];
+                    sh.sum_out[c][p] = sum_out[e][c];
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (uint32_t p = lane; p < (uint32_t)P; p += 256) finish_pixel<B, C, G, F, X, ABS>(k, bx, y, p, sh);
+}
+
 // Workgroup b of nb goes to XCD b % 8 in dispatch order; renumbering the tiles gives each
 // XCD one contiguous run, so neighbouring tiles (which share cache lines at their edges)
 // meet in the same L2.
@@ -816,17 +904,20 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t nb) {
 
 // Waves per SIMD the register allocator is asked to fit without spilling: 3-channel 32-frame
 // tiles fit five (absolute thresholds) or four (relative thresholds: three rank searches),
-// 4-channel tiles one fewer; the 64-frame tiles get the whole register file.
-constexpr int waves_hint(int F, int C, bool absolute) { return F > 32 ? 1 : (absolute ? 5 : 4) - (C > 3 ? 1 : 0); }
+// 4-channel tiles one fewer; two pixels per lane double the frame registers (three waves,
+// two for 4 channels or relative thresholds); the 64-frame tiles get the whole register file.
+constexpr int waves_hint(int F, int C, bool absolute, int X = 1) {
+    return F > 32 ? 1 : X == 2 ? (absolute && C == 3 ? 3 : 2) : (absolute ? 5 : 4) - (C > 3 ? 1 : 0);
+}
 
-template <int B, int C, int G, int F, bool ABS>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(waves_hint(F, C, ABS))))
+template <int B, int C, int G, int F, int X, bool ABS>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(waves_hint(F, C, ABS, X))))
 outlier_fast_kernel(OutlierK k, uint32_t tiles_x) {
-    __shared__ TileShared<C, G, F> sh;
+    __shared__ TileShared<C, G, F, X> sh;
     load_offsets<B, F>(sh.off, G * F, k.pos_off, k.n);
     __syncthreads();
     const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
-    outlier_tile<B, C, G, F, ABS>(k, tile % tiles_x, tile / tiles_x, sh);
+    outlier_tile<B, C, G, F, X, ABS>(k, tile % tiles_x, tile / tiles_x, sh);
 }
 
 }  // namespace cp

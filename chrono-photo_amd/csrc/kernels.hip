@@ -18,7 +18,7 @@ static int launch_outlier_generic_b(const cp_stack* st, const OutlierK& k, hipSt
     return CP_OK;
 }
 
-int launch_outlier(cp_ctx* ctx, const cp_stack* st, const OutlierK& k, uint32_t flags,
+int launch_outlier(cp_ctx* ctx, const cp_stack* st, const OutlierK& k, uint64_t max_pos, uint32_t flags,
                    hipStream_t s) {
     (void)ctx;
     // The register-resident kernel needs the full frame set as the median sample and exact
@@ -29,7 +29,9 @@ int launch_outlier(cp_ctx* ctx, const cp_stack* st, const OutlierK& k, uint32_t 
                          (!sums || exact_sums);
     bool done = false;
     if (fast_ok) {
-        const int shape = ctx->shape_for(st->bps, st->channels, k.n);
+        int shape = ctx->shape_for(st->bps, st->channels, k.n);
+        if (shape >= 0 && kShapes[shape].X == 2 && !pair_reads_fit(st->alloc_bytes, st->bps, st->channels, k, max_pos))
+            shape = narrow_shape(shape);
         if (shape >= 0) done = launch_outlier_fast(st->bps, st->channels, k, shape, s);
     }
     if (!done) {

@@ -11,15 +11,21 @@
 namespace cp {
 
 struct TileShape {
-    int G, F;
+    int G, F, X;
 };
 
-constexpr int kNumShapes = 10;
-constexpr TileShape kShapes[kNumShapes] = {{1, 16}, {1, 32}, {2, 16}, {2, 32}, {4, 16},
-                                           {4, 32}, {8, 16}, {8, 32}, {4, 64}, {8, 64}};
+// Shapes 10-14 put two pixels in each lane (16-bit stacks with 3 or 4 channels): one 12- or
+// 16-byte load per frame instead of one per pixel, which is what lets a wave's loads reach
+// HBM speed; narrow_shape gives the one-pixel shape with the same blocks.
+constexpr int kNumShapes = 15;
+constexpr TileShape kShapes[kNumShapes] = {{1, 16, 1}, {1, 32, 1}, {2, 16, 1}, {2, 32, 1}, {4, 16, 1},
+                                           {4, 32, 1}, {8, 16, 1}, {8, 32, 1}, {4, 64, 1}, {8, 64, 1},
+                                           {1, 16, 2}, {1, 32, 2}, {2, 32, 2}, {4, 32, 2}, {8, 32, 2}};
 constexpr int kNumBuckets = 6;
 constexpr uint32_t kBucketMax[kNumBuckets] = {16, 32, 64, 128, 256, 512};
-constexpr int kBucketCand[kNumBuckets][2] = {{0, -1}, {1, 2}, {3, 4}, {5, 6}, {7, 8}, {9, -1}};
+// Candidates per bucket: [sample bytes - 1][bucket][2]
+constexpr int kBucketCand[2][kNumBuckets][2] = {{{0, -1}, {1, 2}, {3, 4}, {5, 6}, {7, 8}, {9, -1}},
+                                                {{10, 0}, {11, 1}, {12, 3}, {13, 5}, {14, 7}, {9, -1}}};
 // Shapes the batched kernels carry (F <= 32 keeps their register budget).
 constexpr int kBatchShapeMax = 7;
 
@@ -29,30 +35,57 @@ inline int bucket_for(uint32_t n) {
     return -1;
 }
 
-template <int B, int C, int G, int F>
+inline int narrow_shape(int shape) {
+    for (int i = 0; i < kNumShapes; ++i)
+        if (kShapes[i].X == 1 && kShapes[i].G == kShapes[shape].G && kShapes[i].F == kShapes[shape].F) return i;
+    return -1;
+}
+
+template <int B, int C, int G, int F, int X>
 inline void launch_fast_shape(const OutlierK& k, hipStream_t s) {
-    constexpr int P = 256 / G;
+    constexpr int P = X * 256 / G;
     const uint32_t tiles_x = (k.out_w + P - 1) / P;
     dim3 grid(tiles_x * k.out_h), block(256);
-    if (k.th.absolute) hipLaunchKernelGGL((outlier_fast_kernel<B, C, G, F, true>), grid, block, 0, s, k, tiles_x);
-    else hipLaunchKernelGGL((outlier_fast_kernel<B, C, G, F, false>), grid, block, 0, s, k, tiles_x);
+    if (k.th.absolute) hipLaunchKernelGGL((outlier_fast_kernel<B, C, G, F, X, true>), grid, block, 0, s, k, tiles_x);
+    else hipLaunchKernelGGL((outlier_fast_kernel<B, C, G, F, X, false>), grid, block, 0, s, k, tiles_x);
 }
 
 template <int B, int C>
 inline bool launch_fast_c(const OutlierK& k, int shape, hipStream_t s) {
     switch (shape) {
-    case 0: launch_fast_shape<B, C, 1, 16>(k, s); return true;
-    case 1: launch_fast_shape<B, C, 1, 32>(k, s); return true;
-    case 2: launch_fast_shape<B, C, 2, 16>(k, s); return true;
-    case 3: launch_fast_shape<B, C, 2, 32>(k, s); return true;
-    case 4: launch_fast_shape<B, C, 4, 16>(k, s); return true;
-    case 5: launch_fast_shape<B, C, 4, 32>(k, s); return true;
-    case 6: launch_fast_shape<B, C, 8, 16>(k, s); return true;
-    case 7: launch_fast_shape<B, C, 8, 32>(k, s); return true;
-    case 8: launch_fast_shape<B, C, 4, 64>(k, s); return true;
-    case 9: launch_fast_shape<B, C, 8, 64>(k, s); return true;
-    default: return false;
+    case 0: launch_fast_shape<B, C, 1, 16, 1>(k, s); return true;
+    case 1: launch_fast_shape<B, C, 1, 32, 1>(k, s); return true;
+    case 2: launch_fast_shape<B, C, 2, 16, 1>(k, s); return true;
+    case 3: launch_fast_shape<B, C, 2, 32, 1>(k, s); return true;
+    case 4: launch_fast_shape<B, C, 4, 16, 1>(k, s); return true;
+    case 5: launch_fast_shape<B, C, 4, 32, 1>(k, s); return true;
+    case 6: launch_fast_shape<B, C, 8, 16, 1>(k, s); return true;
+    case 7: launch_fast_shape<B, C, 8, 32, 1>(k, s); return true;
+    case 8: launch_fast_shape<B, C, 4, 64, 1>(k, s); return true;
+    case 9: launch_fast_shape<B, C, 8, 64, 1>(k, s); return true;
+    default: break;
     }
+    if constexpr (B == 2) {
+        switch (shape) {
+        case 10: launch_fast_shape<B, C, 1, 16, 2>(k, s); return true;
+        case 11: launch_fast_shape<B, C, 1, 32, 2>(k, s); return true;
+        case 12: launch_fast_shape<B, C, 2, 32, 2>(k, s); return true;
+        case 13: launch_fast_shape<B, C, 4, 32, 2>(k, s); return true;
+        case 14: launch_fast_shape<B, C, 8, 32, 2>(k, s); return true;
+        default: break;
+        }
+    }
+    return false;
+}
+
+// Whether every read of a two-pixel tile over this composite stays inside the stack
+// allocation: the last pixel pair of the last row, at the largest frame offset, reads 16
+// bytes from its first sample.
+inline bool pair_reads_fit(uint64_t alloc_bytes, uint32_t bps, uint32_t channels, const OutlierK& k,
+                           uint64_t max_pos) {
+    const uint64_t x_last = (uint64_t)((k.out_w - 1) / 2) * 2;
+    const uint64_t first = max_pos + ((uint64_t)(k.out_h - 1) * k.stack_w + x_last) * channels;
+    return first * bps + 16 <= alloc_bytes;
 }
 
 // One translation unit per (sample bytes, channels): kernels_fast_b<B>_c<C>.hip.

@@ -60,11 +60,16 @@ struct cp_ctx {
     void uncharge(uint64_t bytes) { used = used >= bytes ? used - bytes : 0; }
 };
 
+// Bytes allocated past the last frame of an owned stack: the two-pixel outlier tiles read up
+// to 16 bytes from a pixel pair's first sample.
+constexpr uint64_t kStackSlack = 256;
+
 struct cp_stack {
     cp_ctx* ctx = nullptr;
     uint32_t width = 0, height = 0, channels = 0, bps = 0;
     uint32_t capacity = 0, count = 0;
     uint8_t* dev = nullptr;
+    uint64_t alloc_bytes = 0;  // readable bytes from dev (owned stacks carry kStackSlack more)
     bool owned = false;
     uint64_t frame_bytes() const { return (uint64_t)width * height * channels * bps; }
     uint64_t frame_elems() const { return (uint64_t)width * height * channels; }
@@ -94,7 +99,8 @@ int prepare_outlier(const cp_stack* st, const cp_frames* frames, const cp_outlie
 FadeK fade_to_device(cp_ctx* ctx, const cp_fade* f, int* rc);
 
 // Kernel launchers (defined in the kernel translation units).
-int launch_outlier(cp_ctx* ctx, const cp_stack* st, const OutlierK& k, uint32_t flags,
+// max_pos: the largest element offset in the composite's frame table.
+int launch_outlier(cp_ctx* ctx, const cp_stack* st, const OutlierK& k, uint64_t max_pos, uint32_t flags,
                    hipStream_t s);
 int launch_simple(cp_ctx* ctx, const cp_stack* st, const SimpleK& k, hipStream_t s);
 

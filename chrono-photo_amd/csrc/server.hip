@@ -7,6 +7,7 @@
 // event) and retires finished requests by polling their events.  Admitted requests run
 // concurrently on different streams; each request's arithmetic is exactly that of a plain
 // call, so results do not depend on what else is in flight.
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -129,7 +130,8 @@ int cp_server::start(Request* r, Slot& slot) {
         k.out_mask = mask;
         k.warnings = (unsigned long long*)j.out.warnings;
         k.stats = (PixelStats*)j.out.stats;
-        if ((rc = launch_outlier(ctx, st, k, 0, s))) return rc;
+        const uint64_t max_pos = *std::max_element(prep.pos_off.begin(), prep.pos_off.end());
+        if ((rc = launch_outlier(ctx, st, k, max_pos, 0, s))) return rc;
         r->pixel_samples = (uint64_t)k.n * k.out_w * k.out_h;
     } else {
         if ((rc = check_weights(j.weights))) return rc;

@@ -220,7 +220,8 @@ CP_API int cp_batch_run(cp_ctx* ctx, const cp_job* jobs, uint32_t n_jobs, uint32
 /* Autotuner: for each frame-count bucket of the register-resident outlier kernel
  * (<=16, 32, 64, 128, 256, 512 frames) time the candidate tile shapes on a synthetic stack
  * of this sample width and channel count and keep the fastest for the context.  shapes
- * (nullable, 6 entries) receives (lanes_per_pixel << 8) | frames_per_lane per bucket. */
+ * (nullable, 6 entries) receives (pixels_per_lane << 16) | (lanes_per_pixel << 8) |
+ * frames_per_lane per bucket. */
 CP_API int cp_autotune(cp_ctx* ctx, uint32_t bytes_per_sample, uint32_t channels, int32_t* shapes);
 
 /* -------------------------------------------------------------------- serving

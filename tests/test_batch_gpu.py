@@ -107,8 +107,8 @@ def test_batch_against_oracle(ctx):
 def test_autotune_keeps_results(ctx, bps):
     shapes = cpa.autotune(ctx, bps, 3)
     assert len(shapes) == 6
-    for (g, f), nmax in zip(shapes, (16, 32, 64, 128, 256, 512)):
-        assert g * f >= nmax and g in (1, 2, 4, 8)
+    for (g, f, xp), nmax in zip(shapes, (16, 32, 64, 128, 256, 512)):
+        assert g * f >= nmax and g in (1, 2, 4, 8) and xp in ((1,) if bps == 1 else (1, 2))
     for n in (12, 30, 60, 120, 250):
         x, st = scene_stack(ctx, 7 + n, n, 70, 24, 3, bps)
         proc = cpa.OutlierProcessor(cpa.Threshold.abs(0.05, 0.2), 0, 4)

@@ -66,6 +66,26 @@ def assert_same(gpu, ref, ctx_msg=""):
     assert warn == owarn, ctx_msg + " warnings"
 
 
+@pytest.mark.parametrize("c,w", [(3, 20), (3, 21), (4, 20), (4, 21)])
+def test_outlier_u16_wrapped_stack(ctx, c, w):
+    """A borrowed allocation ends at its last frame: the tiles that read a pixel pair per
+    load fall back to one pixel per lane where a pair read would pass that end."""
+    from chrono_photo_amd import hip as H
+    n, h = 40, 9
+    x = make_stack(900 + 10 * c + w, n, h, w, c, bits=16)
+    buf = H.DeviceBuffer(x.nbytes)
+    buf.from_host(np.ascontiguousarray(x))
+    st = cpa.Stack(ctx, w, h, c, 2, device_ptr=buf.ptr, n_frames=n)
+    for sel in (2, 4):
+        proc = cpa.OutlierProcessor(cpa.Threshold.abs(0.05, 0.2), 0, sel)
+        img, mask = proc.process(st)
+        oimg, omask, _ = O.outlier(x, O.threshold(True, 0.05, 0.2, 2), 0, sel)
+        assert np.array_equal(img, oimg), (c, w, sel)
+        assert np.array_equal(mask, omask), (c, w, sel)
+    st.close()
+    buf.free()
+
+
 MODES = list(itertools.product([0, 1, 2, 3], [0, 1, 2, 3, 4, 5]))
 
 
@@ -98,7 +118,8 @@ def test_outlier_channel_counts_and_weights(ctx, c):
         assert_same(gpu, ref, f"c={c} w={w} rel")
 
 
-@pytest.mark.parametrize("bits,c,n", [(16, 3, 200), (8, 3, 120), (8, 4, 60), (16, 4, 30)])
+@pytest.mark.parametrize("bits,c,n", [(16, 3, 200), (8, 3, 120), (8, 4, 60), (16, 4, 30), (16, 3, 30),
+                                      (16, 3, 100), (16, 4, 60), (16, 3, 12)])
 def test_outlier_many_tiles_with_crops(ctx, bits, c, n):
     """Many tiles per row, a last tile per row that is partly past the image, and per-frame
     crop origins that put every frame row at a different alignment."""
