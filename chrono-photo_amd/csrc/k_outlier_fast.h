@@ -336,48 +336,66 @@ __device__ __forceinline__ void load_offsets(uint64_t* off, uint32_t count, cons
     for (uint32_t i = threadIdx.x; i < count; i += 256) off[i + 2u * (i / (uint32_t)F)] = pos_off[min(i, n - 1)] * B;
 }
 
-// The 12 bytes at p (2-byte aligned) as three dwords: one 16-byte load from the dword
-// boundary at or below p, realigned by the byte shift (0 or 2).  Reads at most 4 bytes
-// past p + 12.
-__device__ __forceinline__ void load12_a2(const uint8_t* p, uint32_t (&d)[3]) {
-    const uint32_t s = (uint32_t)(uintptr_t)p & 3u;
-    uint32_t q[4];
-    __builtin_memcpy(q, __builtin_assume_aligned(p - s, 4), 16);
-    d[0] = __builtin_amdgcn_alignbyte(q[1], q[0], s);
-    d[1] = __builtin_amdgcn_alignbyte(q[2], q[1], s);
-    d[2] = __builtin_amdgcn_alignbyte(q[3], q[2], s);
-}
-
 // Loads the F frames of one lane block into raw[e*C + c][j/2] (pixel e of the lane, frame j
 // in half j&1); base is the lane's first pixel in stack slot 0 and off the block's F frame
-// byte offsets.  Two pixels per lane (16-bit stacks) take one 12- or 16-byte load per frame.
+// byte offsets.
+// Two pixels per lane (16-bit stacks) take one 16-byte load per frame from the dword
+// boundary at or below the pair's first sample (3 channels: the pair's 12 bytes start 2-byte
+// aligned, so the load reads up to 4 bytes past them and v_alignbyte realigns by the address'
+// low bits).  All F loads are issued before the first is used, so a lane keeps its whole
+// block in flight; the byte shifts are recomputed from the offset table afterwards rather
+// than held in registers meanwhile.
 template <int B, int C, int F, int X>
 __device__ __forceinline__ void load_block(const uint8_t* __restrict__ base, const uint64_t* off,
                                            uint32_t (&raw)[X * C][F / 2]) {
     static_assert(X == 1 || (X == 2 && B == 2 && C >= 3), "two pixels per lane: 16-bit, 3 or 4 channels");
+    if constexpr (X == 2) {
+        typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+        typedef const __attribute__((address_space(1))) u32x4 gu4;
+        typedef const volatile __attribute__((address_space(3))) uint32_t lu32;
+        uint32_t q[F][4];
+#pragma unroll
+        for (int t = 0; t < F; ++t) {
+            const u32x4 v = *(gu4*)(((uintptr_t)base + off[t]) & ~(uintptr_t)3);
+            q[t][0] = v.x;
+            q[t][1] = v.y;
+            q[t][2] = v.z;
+            q[t][3] = v.w;
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        lu32* off32 = (lu32*)off;
+#pragma unroll
+        for (int i = 0; i < F / 2; ++i) {
+            uint32_t a[4], b[4];
+            if constexpr (C == 3) {
+                // v_alignbyte uses the low two bits of the shift operand: the address' low bits
+                const uint32_t s0 = (uint32_t)(uintptr_t)base + off32[4 * i];
+                const uint32_t s1 = (uint32_t)(uintptr_t)base + off32[4 * i + 2];
+#pragma unroll
+                for (int h = 0; h < 3; ++h) {  // [c0|c1], [c2|c0'], [c1'|c2'] of pixels 0 and 1
+                    a[h] = __builtin_amdgcn_alignbyte(q[2 * i][h + 1], q[2 * i][h], s0);
+                    b[h] = __builtin_amdgcn_alignbyte(q[2 * i + 1][h + 1], q[2 * i + 1][h], s1);
+                }
+            } else {
+#pragma unroll
+                for (int h = 0; h < 4; ++h) {
+                    a[h] = q[2 * i][h];
+                    b[h] = q[2 * i + 1][h];
+                }
+            }
+#pragma unroll
+            for (int h = 0; h < C; ++h) {
+                raw[2 * h][i] = __builtin_amdgcn_perm(b[h], a[h], 0x05040100u);
+                raw[2 * h + 1][i] = __builtin_amdgcn_perm(b[h], a[h], 0x07060302u);
+            }
+        }
+        return;
+    }
 #pragma unroll
     for (int i = 0; i < F / 2; ++i) {
         const uint8_t* p0 = base + off[2 * i];
         const uint8_t* p1 = base + off[2 * i + 1];
-        if constexpr (X == 2 && C == 3) {
-            uint32_t a[3], b[3];  // [c0|c1], [c2|c0'], [c1'|c2'] of pixels 0 and 1
-            load12_a2(p0, a);
-            load12_a2(p1, b);
-#pragma unroll
-            for (int h = 0; h < 3; ++h) {
-                raw[2 * h][i] = __builtin_amdgcn_perm(b[h], a[h], 0x05040100u);
-                raw[2 * h + 1][i] = __builtin_amdgcn_perm(b[h], a[h], 0x07060302u);
-            }
-        } else if constexpr (X == 2 && C == 4) {
-            uint32_t a[4], b[4];
-            __builtin_memcpy(a, __builtin_assume_aligned(p0, 8), 16);
-            __builtin_memcpy(b, __builtin_assume_aligned(p1, 8), 16);
-#pragma unroll
-            for (int h = 0; h < 4; ++h) {
-                raw[2 * h][i] = __builtin_amdgcn_perm(b[h], a[h], 0x05040100u);
-                raw[2 * h + 1][i] = __builtin_amdgcn_perm(b[h], a[h], 0x07060302u);
-            }
-        } else if constexpr (B == 2 && C == 3) {
+        if constexpr (B == 2 && C == 3) {
             uint32_t d0, d1;  // channels 0-1 of each frame (2-byte aligned dword)
             __builtin_memcpy(&d0, __builtin_assume_aligned(p0, 2), 4);
             __builtin_memcpy(&d1, __builtin_assume_aligned(p1, 2), 4);
