@@ -310,6 +310,12 @@ struct TileShared {
     static constexpr int MW = F > 32 ? 2 : 1;  // mask words per lane and pixel
     static constexpr int OS = F + 2;           // offset-table stride of a lane block
     uint64_t off[G * OS];      // frame byte offsets at oidx(t); positions >= N repeat N-1
+    // Two pixels per lane: with r the tile row's first sample address modulo 4 (shared by
+    // every pixel pair of the row, which start 12 or 16 bytes apart), aoff = (off + r) & ~3
+    // and sft = (off + r) & 3, so the lane whose pair starts at base loads 16 bytes at
+    // (base & ~3) + aoff and realigns them by sft.
+    uint64_t aoff[X == 2 ? G * OS : 1];
+    uint32_t sft[X == 2 ? G * OS : 1];
     uint32_t mask[MW][NS];     // [w][slot]: outlier bits of frames g*F + 32w + b
     float lmax[NS];            // [slot]: the largest outlier distance (valid when lmaxi != ~0)
     uint32_t lmaxi[NS];        // [slot]: its first frame
@@ -336,6 +342,20 @@ __device__ __forceinline__ void load_offsets(uint64_t* off, uint32_t count, cons
     for (uint32_t i = threadIdx.x; i < count; i += 256) off[i + 2u * (i / (uint32_t)F)] = pos_off[min(i, n - 1)] * B;
 }
 
+// load_offsets plus the realignment tables of a two-pixel tile (TileShared::aoff, sft); r is
+// the tile row's first sample address modulo 4.
+template <int B, int F>
+__device__ __forceinline__ void load_offsets_pair(uint64_t* off, uint64_t* aoff, uint32_t* sft, uint32_t count,
+                                                  const uint64_t* __restrict__ pos_off, uint32_t n, uint32_t r) {
+    for (uint32_t i = threadIdx.x; i < count; i += 256) {
+        const uint32_t j = i + 2u * (i / (uint32_t)F);
+        const uint64_t o = pos_off[min(i, n - 1)] * B;
+        off[j] = o;
+        aoff[j] = (o + r) & ~(uint64_t)3;
+        sft[j] = (uint32_t)(o + r) & 3u;
+    }
+}
+
 // Loads the F frames of one lane block into raw[e*C + c][j/2] (pixel e of the lane, frame j
 // in half j&1); base is the lane's first pixel in stack slot 0 and off the block's F frame
 // byte offsets.
@@ -343,34 +363,34 @@ __device__ __forceinline__ void load_offsets(uint64_t* off, uint32_t count, cons
 // boundary at or below the pair's first sample (3 channels: the pair's 12 bytes start 2-byte
 // aligned, so the load reads up to 4 bytes past them and v_alignbyte realigns by the address'
 // low bits).  All F loads are issued before the first is used, so a lane keeps its whole
-// block in flight; the byte shifts are recomputed from the offset table afterwards rather
+// block in flight; the byte shifts are read from the tile's shift table afterwards rather
 // than held in registers meanwhile.
 template <int B, int C, int F, int X>
 __device__ __forceinline__ void load_block(const uint8_t* __restrict__ base, const uint64_t* off,
-                                           uint32_t (&raw)[X * C][F / 2]) {
+                                           uint32_t (&raw)[X * C][F / 2], const uint64_t* aoff = nullptr,
+                                           const uint32_t* sft = nullptr) {
     static_assert(X == 1 || (X == 2 && B == 2 && C >= 3), "two pixels per lane: 16-bit, 3 or 4 channels");
     if constexpr (X == 2) {
         typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
         typedef const __attribute__((address_space(1))) u32x4 gu4;
         typedef const volatile __attribute__((address_space(3))) uint32_t lu32;
         uint32_t q[F][4];
+        const uintptr_t base_al = (uintptr_t)base & ~(uintptr_t)3;
 #pragma unroll
         for (int t = 0; t < F; ++t) {
-            const u32x4 v = *(gu4*)(((uintptr_t)base + off[t]) & ~(uintptr_t)3);
+            const u32x4 v = *(gu4*)(base_al + aoff[t]);
             q[t][0] = v.x;
             q[t][1] = v.y;
             q[t][2] = v.z;
             q[t][3] = v.w;
         }
         __builtin_amdgcn_sched_barrier(0);
-        lu32* off32 = (lu32*)off;
+        lu32* sft32 = (lu32*)sft;
 #pragma unroll
         for (int i = 0; i < F / 2; ++i) {
             uint32_t a[4], b[4];
             if constexpr (C == 3) {
-                // v_alignbyte uses the low two bits of the shift operand: the address' low bits
-                const uint32_t s0 = (uint32_t)(uintptr_t)base + off32[4 * i];
-                const uint32_t s1 = (uint32_t)(uintptr_t)base + off32[4 * i + 2];
+                const uint32_t s0 = sft32[2 * i], s1 = sft32[2 * i + 1];
 #pragma unroll
                 for (int h = 0; h < 3; ++h) {  // [c0|c1], [c2|c0'], [c1'|c2'] of pixels 0 and 1
                     a[h] = __builtin_amdgcn_alignbyte(q[2 * i][h + 1], q[2 * i][h], s0);
@@ -467,10 +487,11 @@ __device__ __forceinline__ void finish_pixel(const OutlierK& k, uint32_t bx, uin
     if (x >= k.out_w) return;
     const uint64_t pix = ((uint64_t)y * k.stack_w + x) * C;
     float med[4], iqr_inv[4], wi[4], sg[4];
+    const bool want_iqr = !ABS || k.stats != nullptr;
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
         med[c] = sh.med[c][p];
-        iqr_inv[c] = sh.iqr_inv[c][p];
+        iqr_inv[c] = want_iqr ? sh.iqr_inv[c][p] : 0.0f;
         wi[c] = ABS ? k.w[c] : k.w[c] * iqr_inv[c];
         sg[c] = signum_f(k.w[c]);
     }
@@ -708,7 +729,8 @@ __device__ __forceinline__ void outlier_tile(const OutlierK& k, uint32_t bx, uin
 
         // ---- the lane's frame block; pads above N are set to MAXV
         uint32_t raw[X * C][NW];
-        load_block<B, C, F, X>(k.stack + pix * B, sh.off + g * TS::OS, raw);
+        if constexpr (X == 2) load_block<B, C, F, X>(k.stack + pix * B, sh.off, raw, sh.aoff + g * TS::OS, sh.sft + g * TS::OS);
+        else load_block<B, C, F, X>(k.stack + pix * B, sh.off + g * TS::OS, raw);
         if (N < (uint32_t)(G * F)) {  // grid-uniform: only stacks that do not fill the tile
 #pragma unroll
             for (int i = 0; i < NW; ++i) {
@@ -886,24 +908,31 @@ __device__ __forceinline__ void outlier_tile(const OutlierK& k, uint32_t bx, uin
                     const uint32_t vm = nw >= 32 ? 0xFFFFFFFFu : (nw <= 0 ? 0u : (1u << nw) - 1u);
                     sh.mask[w][sl] = m[e][w] & vm;
                 }
-                sh.lmax[sl] = lmax[e];
-                sh.lmaxi[sl] = lmaxi[e];
+                if (want_max) {
+                    sh.lmax[sl] = lmax[e];
+                    sh.lmaxi[sl] = lmaxi[e];
+                }
             }
         }
         if (g == 0) {
+            // only what the finish reads for these options
+            const bool want_iqr = !ABS || k.stats != nullptr;
+            const bool want_s0 = k.background < 2;
 #pragma unroll
             for (int e = 0; e < X; ++e) {
                 const uint32_t p = pl * X + e;
 #pragma unroll
                 for (int c = 0; c < 4; ++c) {
                     sh.med[c][p] = med[e][c];
-                    sh.iqr_inv[c][p] = iqr_inv[e][c];
+                    if (want_iqr) sh.iqr_inv[c][p] = iqr_inv[e][c];
                 }
 #pragma unroll
                 for (int c = 0; c < C; ++c) {
-                    sh.s0[c][p] = raw[e * C + c][0] & 0xFFFFu;
-                    sh.sum_all[c][p] = sum_all[e][c];
-                    sh.sum_out[c][p] = sum_out[e][c];
+                    if (want_s0) sh.s0[c][p] = raw[e * C + c][0] & 0xFFFFu;
+                    if (want_sums) {
+                        sh.sum_all[c][p] = sum_all[e][c];
+                        sh.sum_out[c][p] = sum_out[e][c];
+                    }
                 }
             }
         }
@@ -932,10 +961,16 @@ template <int B, int C, int G, int F, int X, bool ABS>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(waves_hint(F, C, ABS, X))))
 outlier_fast_kernel(OutlierK k, uint32_t tiles_x) {
     __shared__ TileShared<C, G, F, X> sh;
-    load_offsets<B, F>(sh.off, G * F, k.pos_off, k.n);
-    __syncthreads();
     const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
-    outlier_tile<B, C, G, F, X, ABS>(k, tile % tiles_x, tile / tiles_x, sh);
+    const uint32_t y = tile / tiles_x;
+    if constexpr (X == 2) {
+        const uint32_t r = (uint32_t)((uintptr_t)k.stack + (uint64_t)y * k.stack_w * C * B) & 3u;
+        load_offsets_pair<B, F>(sh.off, sh.aoff, sh.sft, G * F, k.pos_off, k.n, r);
+    } else {
+        load_offsets<B, F>(sh.off, G * F, k.pos_off, k.n);
+    }
+    __syncthreads();
+    outlier_tile<B, C, G, F, X, ABS>(k, tile % tiles_x, y, sh);
 }
 
 }  // namespace cp

@@ -30,7 +30,8 @@ int launch_outlier(cp_ctx* ctx, const cp_stack* st, const OutlierK& k, uint64_t 
     bool done = false;
     if (fast_ok) {
         int shape = ctx->shape_for(st->bps, st->channels, k.n);
-        if (shape >= 0 && kShapes[shape].X == 2 && !pair_reads_fit(st->alloc_bytes, st->bps, st->channels, k, max_pos))
+        if (shape >= 0 && kShapes[shape].X == 2 &&
+            !pair_tile_fits(st->dev, st->alloc_bytes, st->bps, st->channels, k, max_pos))
             shape = narrow_shape(shape);
         if (shape >= 0) done = launch_outlier_fast(st->bps, st->channels, k, shape, s);
     }

@@ -78,11 +78,13 @@ inline bool launch_fast_c(const OutlierK& k, int shape, hipStream_t s) {
     return false;
 }
 
-// Whether every read of a two-pixel tile over this composite stays inside the stack
-// allocation: the last pixel pair of the last row, at the largest frame offset, reads 16
-// bytes from its first sample.
-inline bool pair_reads_fit(uint64_t alloc_bytes, uint32_t bps, uint32_t channels, const OutlierK& k,
-                           uint64_t max_pos) {
+// Whether a two-pixel tile may run over this composite: the stack starts on an 8-byte
+// boundary (4-channel pairs are loaded as they lie, 3-channel pairs realigned from dword
+// boundaries), and every read stays inside the stack allocation: the last pixel pair of the
+// last row, at the largest frame offset, reads 16 bytes from its first sample.
+inline bool pair_tile_fits(const void* stack, uint64_t alloc_bytes, uint32_t bps, uint32_t channels,
+                           const OutlierK& k, uint64_t max_pos) {
+    if ((uintptr_t)stack & 7u) return false;
     const uint64_t x_last = (uint64_t)((k.out_w - 1) / 2) * 2;
     const uint64_t first = max_pos + ((uint64_t)(k.out_h - 1) * k.stack_w + x_last) * channels;
     return first * bps + 16 <= alloc_bytes;

@@ -66,16 +66,17 @@ def assert_same(gpu, ref, ctx_msg=""):
     assert warn == owarn, ctx_msg + " warnings"
 
 
-@pytest.mark.parametrize("c,w", [(3, 20), (3, 21), (4, 20), (4, 21)])
-def test_outlier_u16_wrapped_stack(ctx, c, w):
-    """A borrowed allocation ends at its last frame: the tiles that read a pixel pair per
-    load fall back to one pixel per lane where a pair read would pass that end."""
+@pytest.mark.parametrize("c,w,shift", [(3, 20, 0), (3, 21, 0), (4, 20, 0), (4, 21, 0), (3, 20, 2), (4, 21, 2)])
+def test_outlier_u16_wrapped_stack(ctx, c, w, shift):
+    """A borrowed allocation ends at its last frame and may start off an 8-byte boundary: the
+    tiles that read a pixel pair per load fall back to one pixel per lane where a pair read
+    would pass that end or start misaligned."""
     from chrono_photo_amd import hip as H
     n, h = 40, 9
-    x = make_stack(900 + 10 * c + w, n, h, w, c, bits=16)
-    buf = H.DeviceBuffer(x.nbytes)
-    buf.from_host(np.ascontiguousarray(x))
-    st = cpa.Stack(ctx, w, h, c, 2, device_ptr=buf.ptr, n_frames=n)
+    x = np.ascontiguousarray(make_stack(900 + 10 * c + w, n, h, w, c, bits=16))
+    buf = H.DeviceBuffer(x.nbytes + shift)
+    H.check(H.hip().hipMemcpy(buf.ptr + shift, x.ctypes.data, x.nbytes, H.hipMemcpyHostToDevice), "hipMemcpy")
+    st = cpa.Stack(ctx, w, h, c, 2, device_ptr=buf.ptr + shift, n_frames=n)
     for sel in (2, 4):
         proc = cpa.OutlierProcessor(cpa.Threshold.abs(0.05, 0.2), 0, sel)
         img, mask = proc.process(st)
@@ -84,6 +85,22 @@ def test_outlier_u16_wrapped_stack(ctx, c, w):
         assert np.array_equal(mask, omask), (c, w, sel)
     st.close()
     buf.free()
+
+
+@pytest.mark.parametrize("n", [16, 32, 64, 128, 256])
+@pytest.mark.parametrize("c", [3, 4])
+def test_outlier_u16_full_tiles(ctx, n, c):
+    """Frame counts that fill the tile exactly take the 7-bit window search around the mean
+    estimate; noisy pixels, pixels with runs of outliers and constant pixels all stay exact."""
+    for frac, seed in ((0.02, 1), (0.3, 2)):
+        x = make_stack(1300 + 10 * n + c + seed, n, 5, 70, c, bits=16, outlier_frac=frac)
+        x[:, 1, 2:6, :] = 40000                        # constant pixels
+        x[: n // 3, 2, 2:6, :] = 1000                  # a run of outliers in the first blocks
+        x[:, 3, 2:6, :] = 65535 - (np.arange(n) % 3)[:, None, None]  # at the top of the range
+        x[:, 4, 2:6, :] = np.arange(n)[:, None, None] % 2            # at the bottom
+        for sel in (2, 4):
+            gpu, ref = run_both(ctx, x, True, 0.05, 0.2, 0, sel)
+            assert_same(gpu, ref, f"n={n} c={c} frac={frac} sel={sel}")
 
 
 MODES = list(itertools.product([0, 1, 2, 3], [0, 1, 2, 3, 4, 5]))
