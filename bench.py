@@ -27,6 +27,23 @@ sys.path[:0] = [os.path.join(ROOT, "chrono-photo_amd")]
 HBM_PEAK_GBS = 8000.0
 
 
+def measured_traffic(workload):
+    """HBM bytes per launch of the workload's dominant kernel, as recorded in its committed
+    rocprofv3 summary (FETCH_SIZE and WRITE_SIZE passes scaled by a plain copy's counts):
+    the `traffic_bytes_per_launch:` line of profiles/<kernel>__<workload>.md, else None."""
+    pdir = os.path.join(ROOT, "profiles")
+    if not os.path.isdir(pdir):
+        return None
+    for name in sorted(os.listdir(pdir)):
+        if not name.endswith(f"__{workload}.md") or name.startswith("shake_"):
+            continue
+        with open(os.path.join(pdir, name)) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return {"bytes": int(line.split(":")[1]), "source": f"profiles/{name}"}
+    return None
+
+
 WORKLOADS = {
     "outlier_4k_rgb16_256f": dict(
         scene=dict(seed=2, width=3840, height=2160, channels=3, bytes_per_sample=2, n_frames=256,
@@ -363,6 +380,7 @@ def main():
         print(f"[bench] serving: {extra['serving']['speedup']:.2f}x over plain, p50 "
               f"{extra['serving']['p50_ms']:.2f} ms p99 {extra['serving']['p99_ms']:.2f} ms "
               f"({time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
+    traffic = measured_traffic(args.workload)
     res = {
         "metric": "pixel-samples/s outlier mode (% HBM roofline)" if wl["mode"] == "outlier"
         else f"pixel-samples/s {wl['mode']} mode (% HBM roofline)",
@@ -375,7 +393,9 @@ def main():
                        "outlier": wl["outlier"]} if wl["mode"] == "outlier" else {}),
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved_gbs / HBM_PEAK_GBS,
+                     "traffic": traffic["bytes"] if traffic else None,
+                     **({"traffic_source": traffic["source"]} if traffic else {}),
                      "algorithmic_bytes_per_launch": alg_bytes},
         "cpu_baseline": cpu,
         **extra,
