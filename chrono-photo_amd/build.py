@@ -31,7 +31,7 @@ LIBS = {
     "libchrono_hip.so": ["abi.hip", "kernels.hip", "kernels_fast_b1_c3.hip", "kernels_fast_b1_c4.hip",
                          "kernels_fast_b2_c3.hip", "kernels_fast_b2_c4.hip", "kernels_batch_b1_c3.hip",
                          "kernels_batch_b1_c4.hip",
-                         "shake.hip", "batch.hip", "server.hip"],
+                         "shake.hip", "batch.hip", "server.hip", "outofcore.hip"],
     "libchrono_synth.so": ["synth.hip"],
 }
 

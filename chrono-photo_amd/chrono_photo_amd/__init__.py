@@ -141,7 +141,7 @@ ABI_SYMBOLS = [
     "cp_ctx_stream", "cp_stack_create", "cp_stack_wrap", "cp_stack_upload", "cp_stack_push",
     "cp_stack_commit", "cp_stack_download",
     "cp_stack_info_get", "cp_stack_destroy", "cp_threshold_new", "cp_threshold_blend_value",
-    "cp_fade_table", "cp_fade_get", "cp_outlier", "cp_simple", "cp_shake", "cp_crop_create",
+    "cp_fade_table", "cp_fade_get", "cp_outlier", "cp_outlier_host", "cp_simple", "cp_shake", "cp_crop_create",
     "cp_video_windows", "cp_batch_run", "cp_batch_create", "cp_batch_launch", "cp_batch_destroy",
     "cp_autotune", "cp_server_create", "cp_server_submit", "cp_server_poll",
     "cp_server_wait", "cp_server_stats_get", "cp_server_destroy",
@@ -190,6 +190,8 @@ def load_library() -> C.CDLL:
         "cp_fade_get": (C.c_float, [P(cp_fade), C.c_int32]),
         "cp_outlier": (C.c_int, [C.c_void_p, C.c_void_p, P(cp_frames), P(cp_outlier_params),
                                  P(cp_outputs), C.c_uint32]),
+        "cp_outlier_host": (C.c_int, [C.c_void_p, P(C.c_void_p), C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                      C.c_uint32, P(cp_frames), P(cp_outlier_params), P(cp_outputs), C.c_uint64]),
         "cp_simple": (C.c_int, [C.c_void_p, C.c_void_p, P(cp_frames), P(C.c_float), P(cp_fade),
                                 C.c_int32, C.c_void_p, C.c_uint32]),
         "cp_shake": (C.c_int, [C.c_void_p, C.c_void_p, P(C.c_uint32), C.c_uint32, P(C.c_int32),
@@ -642,6 +644,40 @@ class OutlierProcessor:
         flags = CP_FLAG_FORCE_GENERIC if force_generic else 0
         _check(stack.lib.cp_outlier(stack.ctx.handle, stack.handle, C.byref(sel.c), C.byref(p),
                                     C.byref(out), flags))
+        self.warnings = int(warn[0])
+        res = (img, mask)
+        if with_stats:
+            res = res + (stats,)
+        return res
+
+    def process_host(self, ctx: "Context", frames, image_indices=None, crop=None, with_mask: bool = True,
+                     with_stats: bool = False, band_bytes: int = 0):
+        """Out-of-core composite of frames held in host memory (cp_outlier_host): `frames` is
+        an array [n][h][w][c] or a sequence of [h][w][c] arrays (u8 or u16); row bands stream
+        through a device band stack of at most `band_bytes` (0: what the context allows).
+        Returns the same as process() over a stack of these frames."""
+        arrs = [np.ascontiguousarray(f) for f in frames]
+        if not arrs:
+            raise ValueError("no frames")
+        h0, w0, c0 = arrs[0].shape
+        dtype = arrs[0].dtype
+        if dtype not in (np.uint8, np.uint16) or any(a.shape != (h0, w0, c0) or a.dtype != dtype for a in arrs):
+            raise ValueError("frames must share one [h][w][c] u8 or u16 layout")
+        ptrs = (C.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+        n = len(arrs) if image_indices is None else None
+        sel = _FrameSel(None, image_indices, crop, n=n)
+        h = sel.out_h or h0
+        w = sel.out_w or w0
+        img = np.zeros((h, w, c0), dtype)
+        mask = np.zeros((h, w, c0), np.uint8) if with_mask else None
+        stats = np.zeros((h, w), PIXEL_STATS_DTYPE) if with_stats else None
+        warn = np.zeros(1, np.uint64)
+        out = cp_outputs(img.ctypes.data, mask.ctypes.data if mask is not None else None,
+                         warn.ctypes.data, stats.ctypes.data if stats is not None else None)
+        bps = dtype.itemsize
+        p = self.params(bps)
+        _check(ctx.lib.cp_outlier_host(ctx.handle, ptrs, len(arrs), w0, h0, c0, bps, C.byref(sel.c),
+                                       C.byref(p), C.byref(out), band_bytes))
         self.warnings = int(warn[0])
         res = (img, mask)
         if with_stats:

@@ -570,10 +570,10 @@ __device__ __forceinline__ void finish_pixel(const OutlierK& k, uint32_t bx, uin
             else bg_index = num == 0 ? 0 : first_non;
         } else {
             if (num == 0 || num == N) {
-                bg_index = rng_range(k.seed, pix_id, 0, N);
+                bg_index = rng_range(k.seed, k.pix_base + pix_id, 0, N);
                 warning = num == N;
             } else {
-                uint32_t pos = rng_range(k.seed, pix_id, 0, N - num);
+                uint32_t pos = rng_range(k.seed, k.pix_base + pix_id, 0, N - num);
                 uint32_t j = num;
 #pragma unroll
                 for (int q = G - 1; q >= 0; --q) {

@@ -237,14 +237,14 @@ __global__ void __launch_bounds__(256) outlier_generic_kernel(OutlierK k) {
             else bg_index = num == 0 ? 0 : first_non;
         } else {  // Random: sample_excluded src/chrono.rs:532-555
             if (num == 0) {
-                bg_index = rng_range(k.seed, pix_id, 0, N);
+                bg_index = rng_range(k.seed, k.pix_base + pix_id, 0, N);
             } else if (num == N) {
-                bg_index = rng_range(k.seed, pix_id, 0, N);
+                bg_index = rng_range(k.seed, k.pix_base + pix_id, 0, N);
                 warning = true;
             } else {
                 // Position r of the swapped index array, traced back through the swaps
                 // (a_j = j-th outlier, b_j = N - 1 - j) applied in order j = 0..num-1.
-                uint32_t r = rng_range(k.seed, pix_id, 0, N - num);
+                uint32_t r = rng_range(k.seed, k.pix_base + pix_id, 0, N - num);
                 uint32_t pos = r;
                 uint32_t j = num;
                 for (int64_t t = (int64_t)N - 1; t >= 0; --t) {

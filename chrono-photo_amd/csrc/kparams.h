@@ -43,6 +43,7 @@ struct OutlierK {
     float w[4];
     FadeK fade;
     uint64_t seed;
+    uint64_t pix_base;           // image index of output pixel (0, 0): row bands draw the same random numbers
     void* out_px;
     uint8_t* out_mask;
     unsigned long long* warnings;

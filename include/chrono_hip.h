@@ -162,6 +162,19 @@ CP_API float cp_fade_get(const cp_fade* fade, int32_t frame);
 /* OutlierProcessor::process per-pixel path (src/chrono.rs:73-206 + calc_pixel :208-494). */
 CP_API int cp_outlier(cp_ctx* ctx, const cp_stack* st, const cp_frames* frames,
                       const cp_outlier_params* params, const cp_outputs* out, uint32_t flags);
+/* Out-of-core OutlierProcessor::process: the reference composites time slices that its
+ * slicer spilled to disk (src/chrono.rs:96-205 over src/slicer.rs, src/streams.rs:115-219);
+ * here the frames stay in host memory and row bands of every frame position are streamed
+ * through a device band stack, one composite per band.  host_frames[i] points at frame i
+ * ([height][width][channels] samples, bytes_per_sample each); sel->frames (NULL: 0..n-1)
+ * indexes host_frames, crops and frame_offset as for cp_outlier.  The band stack uses at
+ * most band_bytes of device memory (0: what the context's HBM budget leaves, or a quarter
+ * of the free device memory when the context has no budget).  Outputs are host buffers.
+ * Results equal cp_outlier over a stack that holds the same frames. */
+CP_API int cp_outlier_host(cp_ctx* ctx, const void* const* host_frames, uint32_t n_host_frames,
+                           uint32_t width, uint32_t height, uint32_t channels,
+                           uint32_t bytes_per_sample, const cp_frames* sel,
+                           const cp_outlier_params* params, const cp_outputs* out, uint64_t band_bytes);
 /* SimpleProcessor::process (src/simple.rs:26-168): darker != 0 selects `--mode darker`. */
 CP_API int cp_simple(cp_ctx* ctx, const cp_stack* st, const cp_frames* frames,
                      const float weights[4], const cp_fade* fade, int32_t darker, void* out_px,

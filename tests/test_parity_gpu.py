@@ -103,6 +103,40 @@ def test_outlier_u16_full_tiles(ctx, n, c):
             assert_same(gpu, ref, f"n={n} c={c} frac={frac} sel={sel}")
 
 
+@pytest.mark.parametrize("bits,c", [(8, 3), (16, 3), (16, 4), (8, 1)])
+def test_outlier_host_bands_equal_resident(ctx, bits, c):
+    """Out-of-core composites (row bands of host frames streamed through a band stack) equal
+    the resident composite: backgrounds incl. random, outlier modes, crops, a frame
+    selection, statistics and warnings, for bands of 1, 3 and all rows."""
+    n, h, w = 40, 23, 37
+    x = make_stack(1500 + bits + c, n, h, w, c, bits=bits)
+    rng = np.random.default_rng(5)
+    crop_xy = rng.integers(0, 5, (n, 2)).astype(np.int32)
+    crop = (crop_xy, w - 4, h - 4)
+    st = cpa.Stack.from_array(ctx, x)
+    row = w * c * (bits // 8)
+    for bg, sel in ((1, 4), (0, 2), (2, 3), (3, 5)):
+        proc = cpa.OutlierProcessor(cpa.Threshold.abs(0.05, 0.2), bg, sel, seed=9)
+        ref = proc.process(st, crop=crop, with_stats=True)
+        ref_warn = proc.warnings
+        for band_rows in (1, 3, 64):
+            band_bytes = (band_rows + 4) * row * n + 256  # crop halo of at most 4 rows + slack
+            got = proc.process_host(ctx, x, crop=crop, with_stats=True, band_bytes=band_bytes)
+            for a, b, what in zip(got, ref, ("image", "mask", "stats")):
+                assert np.array_equal(a, b), (bits, c, bg, sel, band_rows, what)
+            assert proc.warnings == ref_warn
+    # a frame selection (the reference's image_indices) without crops
+    idx = [3, 4, 9, 10, 11, 20, 31, 32, 38]
+    proc = cpa.OutlierProcessor(cpa.Threshold.abs(0.05, 0.2), 0, 0)
+    ref = proc.process(st, image_indices=idx)
+    got = proc.process_host(ctx, x, image_indices=idx, band_bytes=2 * row * len(idx) + 256)
+    assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
+    with pytest.raises(cpa.ChronoError) as e:
+        proc.process_host(ctx, x, crop=crop, band_bytes=row * n)
+    assert e.value.code == cpa.CP_ERR_OUT_OF_HBM_BUDGET
+    st.close()
+
+
 MODES = list(itertools.product([0, 1, 2, 3], [0, 1, 2, 3, 4, 5]))
 
 
