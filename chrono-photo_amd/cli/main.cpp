@@ -8,11 +8,14 @@
 // then one composite or one per video window (src/main.rs:214-429).  Frames are decoded once
 // into a device-resident stack; the time-slicing flags (--temp-dir, --slice, --compression)
 // and the thread-count flags only tune the reference's disk pipeline and are accepted with a
-// warning.  Images: PNG (8/16-bit) and binary PPM/PGM.
+// warning.  Images: PNG (8/16-bit) and binary PPM/PGM.  One option is this tool's own:
+// --hbm-budget <size>[K|M|G] caps the device memory of the run; a stack larger than that
+// stays in host memory and outlier composites stream it through the GPU in row bands.
 #include <glob.h>
 #include <sys/stat.h>
 
 #include <algorithm>
+#include <cctype>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -75,8 +78,26 @@ struct Args {
     float weights[4] = {1, 1, 1, 1};
     long sample = -1;
     uint64_t seed = 0;
+    uint64_t hbm_budget = 0;  // bytes of device memory the run may use (0: no limit)
     std::vector<std::string> unused;
 };
+
+// "<number>[K|M|G]" in bytes (powers of 1024).
+uint64_t parse_bytes(const std::string& s) {
+    if (s.empty()) die("--hbm-budget expects a size");
+    uint64_t mul = 1;
+    std::string num = s;
+    const char u = (char)toupper((unsigned char)s.back());
+    if (u == 'K' || u == 'M' || u == 'G') {
+        mul = u == 'K' ? 1ull << 10 : u == 'M' ? 1ull << 20 : 1ull << 30;
+        num = s.substr(0, s.size() - 1);
+    }
+    try {
+        return (uint64_t)std::stoull(num) * mul;
+    } catch (...) {
+        die("--hbm-budget expects a size such as 512M or 8G, got " + s);
+    }
+}
 
 // The reference's option-file tokenizer (src/cli.rs:245-266): quotes group, spaces split.
 std::vector<std::string> tokenize_options(const std::string& text) {
@@ -114,6 +135,7 @@ Args parse_args(std::vector<std::string> argv) {
         else if (o == "--fade") a.fade = val();
         else if (o == "--sample") a.sample = std::stol(val());
         else if (o == "--seed") a.seed = std::stoull(val());
+        else if (o == "--hbm-budget") a.hbm_budget = parse_bytes(val());
         else if (o == "--weights") {
             for (int k = 0; k < 4; ++k) a.weights[k] = std::stof(val());
         } else if (o == "--shake") a.shake = val();
@@ -194,24 +216,34 @@ int main(int argc, char** argv) {
     if (!read_image(files[0], &first, &err)) die(err);
 
     cp_ctx* ctx = nullptr;
-    check(cp_ctx_create(0, 0, &ctx), "device");
+    check(cp_ctx_create(0, a.hbm_budget, &ctx), "device");
+    const uint32_t N = (uint32_t)files.size(), W = first.width, H = first.height, C = first.channels,
+                   B = first.bytes_per_sample;
+    // Stacks larger than --hbm-budget stay in host memory and outlier composites stream row
+    // bands through the device (the reference's time slices, src/slicer.rs).
+    const uint64_t stack_bytes = (uint64_t)N * W * H * C * B;
+    const bool out_of_core = a.hbm_budget && stack_bytes + (stack_bytes / 4) > a.hbm_budget;
+    if (out_of_core) {
+        if (a.mode != "outlier") die("--mode " + a.mode + " needs the whole stack on the GPU: raise --hbm-budget");
+        if (!a.shake.empty()) die("--shake needs the whole stack on the GPU: raise --hbm-budget");
+        printf("Stack of %llu bytes exceeds --hbm-budget: compositing row bands from host memory\n",
+               (unsigned long long)stack_bytes);
+    }
     cp_stack* st = nullptr;
-    check(cp_stack_create(ctx, first.width, first.height, first.channels, first.bytes_per_sample,
-                          (uint32_t)files.size(), &st), "stack");
+    std::vector<Image> host;  // out of core: the decoded frames
+    if (!out_of_core)
+        check(cp_stack_create(ctx, W, H, C, B, N, &st), "stack");
     for (size_t i = 0; i < files.size(); ++i) {
         Image img;
         if (i == 0) img = std::move(first);
         else if (!read_image(files[i], &img, &err)) die(err);
-        cp_stack_info info;
-        cp_stack_info_get(st, &info);
-        if (img.width != info.width || img.height != info.height || img.channels != info.channels ||
-            img.bytes_per_sample != info.bytes_per_sample)
+        if (img.width != W || img.height != H || img.channels != C || img.bytes_per_sample != B)
             die("Image layout does not fit! (" + files[i] + ")");
-        check(cp_stack_push(st, img.data.data(), 1, nullptr), "upload");
+        if (out_of_core) host.push_back(std::move(img));
+        else check(cp_stack_push(st, img.data.data(), 1, nullptr), "upload");
     }
-    cp_stack_info info;
-    cp_stack_info_get(st, &info);
-    const uint32_t N = info.count, W = info.width, H = info.height, C = info.channels, B = info.bytes_per_sample;
+    std::vector<const void*> host_ptrs;
+    for (const Image& im : host) host_ptrs.push_back(im.data.data());
     if (a.debug) printf("%u frames %ux%u, %u channels, %u-bit\n", N, W, H, C, 8 * B);
 
     // shake correction, src/main.rs:61-85
@@ -318,7 +350,10 @@ int main(int argc, char** argv) {
             mask.data.assign((size_t)out_w * out_h * C, 0);
             uint64_t warnings = 0;
             cp_outputs o{img.data.data(), mask.data.data(), &warnings, nullptr};
-            check(cp_outlier(ctx, st, &fr, &pp, &o, 0), "outlier composite");
+            if (out_of_core)
+                check(cp_outlier_host(ctx, host_ptrs.data(), N, W, H, C, B, &fr, &pp, &o, 0), "outlier composite");
+            else
+                check(cp_outlier(ctx, st, &fr, &pp, &o, 0), "outlier composite");
             if (warnings) printf("Warning: %llu pixels seem to consist of only outliers\n", (unsigned long long)warnings);
             if (!blend_path.empty() && !write_image(blend_path, mask, &err)) die(err);
         } else {
@@ -358,7 +393,7 @@ int main(int argc, char** argv) {
         for (uint32_t i = 0; i < N; ++i) idx[i] = i;
         composite(idx, a.output, a.output_blend);
     }
-    cp_stack_destroy(st);
+    if (st) cp_stack_destroy(st);
     cp_ctx_destroy(ctx);
     const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
     printf("Total time: %.3fs\n", s);

@@ -230,3 +230,23 @@ def test_cli_video_windows(tmp_path):
             continue
         img, _, _ = orc.outlier(st, thr, 0, 2, slots=slots)
         np.testing.assert_array_equal(read_png(str(tmp_path / f"vid-{k:05d}.png")), img, err_msg=f"window {k}")
+
+
+@pytest.mark.gpu
+def test_cli_out_of_core_matches_resident(tmp_path):
+    """--hbm-budget below the stack size: row bands stream from host memory and the image and
+    blend mask equal the resident run's."""
+    st, pat = _frames(tmp_path, bps=2)
+    outs = {}
+    for name, extra in (("resident", ()), ("banded", ("--hbm-budget", "40K"))):
+        out, blend = str(tmp_path / f"{name}.png"), str(tmp_path / f"{name}_b.png")
+        r = _cli("--pattern", pat, "--output", out, "--output-blend", blend, "--background", "first",
+                 "--outlier", "forward", *extra)
+        assert r.returncode == 0, r.stderr
+        outs[name] = (read_png(out), read_png(blend))
+        if name == "banded":
+            assert "row bands" in r.stdout
+    np.testing.assert_array_equal(outs["banded"][0], outs["resident"][0])
+    np.testing.assert_array_equal(outs["banded"][1], outs["resident"][1])
+    r = _cli("--pattern", pat, "--output", str(tmp_path / "d.png"), "--mode", "darker", "--hbm-budget", "40K")
+    assert r.returncode != 0 and "hbm-budget" in r.stderr
