@@ -141,7 +141,7 @@ ABI_SYMBOLS = [
     "cp_ctx_stream", "cp_stack_create", "cp_stack_wrap", "cp_stack_upload", "cp_stack_push",
     "cp_stack_commit", "cp_stack_download",
     "cp_stack_info_get", "cp_stack_destroy", "cp_threshold_new", "cp_threshold_blend_value",
-    "cp_fade_table", "cp_fade_get", "cp_outlier", "cp_outlier_host", "cp_simple", "cp_shake", "cp_crop_create",
+    "cp_fade_table", "cp_fade_get", "cp_outlier", "cp_outlier_host", "cp_simple", "cp_simple_host", "cp_shake", "cp_crop_create",
     "cp_video_windows", "cp_batch_run", "cp_batch_create", "cp_batch_launch", "cp_batch_destroy",
     "cp_autotune", "cp_server_create", "cp_server_submit", "cp_server_poll",
     "cp_server_wait", "cp_server_stats_get", "cp_server_destroy",
@@ -194,6 +194,9 @@ def load_library() -> C.CDLL:
                                       C.c_uint32, P(cp_frames), P(cp_outlier_params), P(cp_outputs), C.c_uint64]),
         "cp_simple": (C.c_int, [C.c_void_p, C.c_void_p, P(cp_frames), P(C.c_float), P(cp_fade),
                                 C.c_int32, C.c_void_p, C.c_uint32]),
+        "cp_simple_host": (C.c_int, [C.c_void_p, P(C.c_void_p), C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                     C.c_uint32, P(cp_frames), P(C.c_float), P(cp_fade), C.c_int32, C.c_void_p,
+                                     C.c_uint64]),
         "cp_shake": (C.c_int, [C.c_void_p, C.c_void_p, P(C.c_uint32), C.c_uint32, P(C.c_int32),
                                C.c_uint32, C.c_uint32, C.c_uint32, P(C.c_int32)]),
         "cp_crop_create": (C.c_int, [P(C.c_int32), C.c_uint32, C.c_uint32, C.c_uint32, P(C.c_int32),
@@ -601,6 +604,18 @@ class _FrameSel:
         return h, w, stack.channels
 
 
+def _host_frames(frames):
+    """Contiguous host frames of one [h][w][c] u8/u16 layout and a pointer array to them."""
+    arrs = [np.ascontiguousarray(f) for f in frames]
+    if not arrs:
+        raise ValueError("no frames")
+    shape, dtype = arrs[0].shape, arrs[0].dtype
+    if len(shape) != 3 or dtype not in (np.uint8, np.uint16) or any(a.shape != shape or a.dtype != dtype
+                                                                      for a in arrs):
+        raise ValueError("frames must share one [h][w][c] u8 or u16 layout")
+    return arrs, (C.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs]), shape, dtype
+
+
 class OutlierProcessor:
     """OutlierProcessor (src/chrono.rs:32-206) over a device-resident stack."""
 
@@ -656,14 +671,7 @@ class OutlierProcessor:
         an array [n][h][w][c] or a sequence of [h][w][c] arrays (u8 or u16); row bands stream
         through a device band stack of at most `band_bytes` (0: what the context allows).
         Returns the same as process() over a stack of these frames."""
-        arrs = [np.ascontiguousarray(f) for f in frames]
-        if not arrs:
-            raise ValueError("no frames")
-        h0, w0, c0 = arrs[0].shape
-        dtype = arrs[0].dtype
-        if dtype not in (np.uint8, np.uint16) or any(a.shape != (h0, w0, c0) or a.dtype != dtype for a in arrs):
-            raise ValueError("frames must share one [h][w][c] u8 or u16 layout")
-        ptrs = (C.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+        arrs, ptrs, (h0, w0, c0), dtype = _host_frames(frames)
         n = len(arrs) if image_indices is None else None
         sel = _FrameSel(None, image_indices, crop, n=n)
         h = sel.out_h or h0
@@ -701,6 +709,18 @@ class SimpleProcessor:
         _check(stack.lib.cp_simple(stack.ctx.handle, stack.handle, C.byref(sel.c),
                                    _ptr(self.weights, C.c_float), C.byref(f), int(self.darker),
                                    C.c_void_p(img.ctypes.data), 0))
+        return img
+
+    def process_host(self, ctx: "Context", frames, crop=None, image_indices=None, band_bytes: int = 0):
+        """Out-of-core composite of frames held in host memory (cp_simple_host); returns the
+        same image as process() over a stack of these frames."""
+        arrs, ptrs, (h0, w0, c0), dtype = _host_frames(frames)
+        sel = _FrameSel(None, image_indices, crop, n=len(arrs) if image_indices is None else None)
+        img = np.zeros((sel.out_h or h0, sel.out_w or w0, c0), dtype)
+        f = self.fade.native()
+        _check(ctx.lib.cp_simple_host(ctx.handle, ptrs, len(arrs), w0, h0, c0, dtype.itemsize, C.byref(sel.c),
+                                      _ptr(self.weights, C.c_float), C.byref(f), int(self.darker),
+                                      C.c_void_p(img.ctypes.data), band_bytes))
         return img
 
 

@@ -10,7 +10,7 @@
 // and the thread-count flags only tune the reference's disk pipeline and are accepted with a
 // warning.  Images: PNG (8/16-bit) and binary PPM/PGM.  One option is this tool's own:
 // --hbm-budget <size>[K|M|G] caps the device memory of the run; a stack larger than that
-// stays in host memory and outlier composites stream it through the GPU in row bands.
+// stays in host memory and the composites stream it through the GPU in row bands.
 #include <glob.h>
 #include <sys/stat.h>
 
@@ -224,7 +224,6 @@ int main(int argc, char** argv) {
     const uint64_t stack_bytes = (uint64_t)N * W * H * C * B;
     const bool out_of_core = a.hbm_budget && stack_bytes + (stack_bytes / 4) > a.hbm_budget;
     if (out_of_core) {
-        if (a.mode != "outlier") die("--mode " + a.mode + " needs the whole stack on the GPU: raise --hbm-budget");
         if (!a.shake.empty()) die("--shake needs the whole stack on the GPU: raise --hbm-budget");
         printf("Stack of %llu bytes exceeds --hbm-budget: compositing row bands from host memory\n",
                (unsigned long long)stack_bytes);
@@ -357,7 +356,12 @@ int main(int argc, char** argv) {
             if (warnings) printf("Warning: %llu pixels seem to consist of only outliers\n", (unsigned long long)warnings);
             if (!blend_path.empty() && !write_image(blend_path, mask, &err)) die(err);
         } else {
-            check(cp_simple(ctx, st, &fr, a.weights, &fade, a.mode == "darker", img.data.data(), 0), "composite");
+            if (out_of_core)
+                check(cp_simple_host(ctx, host_ptrs.data(), N, W, H, C, B, &fr, a.weights, &fade, a.mode == "darker",
+                                     img.data.data(), 0),
+                      "composite");
+            else
+                check(cp_simple(ctx, st, &fr, a.weights, &fade, a.mode == "darker", img.data.data(), 0), "composite");
         }
         if (!write_image(out_path, img, &err)) die(err);
     };

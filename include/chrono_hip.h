@@ -179,6 +179,12 @@ CP_API int cp_outlier_host(cp_ctx* ctx, const void* const* host_frames, uint32_t
 CP_API int cp_simple(cp_ctx* ctx, const cp_stack* st, const cp_frames* frames,
                      const float weights[4], const cp_fade* fade, int32_t darker, void* out_px,
                      uint32_t flags);
+/* SimpleProcessor::process over frames in host memory, streamed in row bands as for
+ * cp_outlier_host; out_px is a host buffer.  Results equal cp_simple. */
+CP_API int cp_simple_host(cp_ctx* ctx, const void* const* host_frames, uint32_t n_host_frames,
+                          uint32_t width, uint32_t height, uint32_t channels,
+                          uint32_t bytes_per_sample, const cp_frames* sel, const float weights[4],
+                          const cp_fade* fade, int32_t darker, void* out_px, uint64_t band_bytes);
 /* ShakeAnalyzer::analyze (src/shake.rs:190-313) over frame positions; offsets receives
  * (dx, dy) per position, position 0 = (0, 0).  frames may be NULL (slots 0..n-1). */
 CP_API int cp_shake(cp_ctx* ctx, const cp_stack* st, const uint32_t* frames, uint32_t n,

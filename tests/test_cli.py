@@ -248,5 +248,14 @@ def test_cli_out_of_core_matches_resident(tmp_path):
             assert "row bands" in r.stdout
     np.testing.assert_array_equal(outs["banded"][0], outs["resident"][0])
     np.testing.assert_array_equal(outs["banded"][1], outs["resident"][1])
-    r = _cli("--pattern", pat, "--output", str(tmp_path / "d.png"), "--mode", "darker", "--hbm-budget", "40K")
+    for mode in ("darker", "lighter"):
+        dk = {}
+        for name, extra in (("resident", ()), ("banded", ("--hbm-budget", "40K"))):
+            out = str(tmp_path / f"{mode}_{name}.png")
+            r = _cli("--pattern", pat, "--output", out, "--mode", mode, *extra)
+            assert r.returncode == 0, r.stderr
+            dk[name] = read_png(out)
+        np.testing.assert_array_equal(dk["banded"], dk["resident"])
+    r = _cli("--pattern", pat, "--output", str(tmp_path / "s.png"), "--shake", "3/2", "--shake-anchors", "20/20",
+             "--hbm-budget", "40K")
     assert r.returncode != 0 and "hbm-budget" in r.stderr

@@ -137,6 +137,30 @@ def test_outlier_host_bands_equal_resident(ctx, bits, c):
     st.close()
 
 
+@pytest.mark.parametrize("bits,c", [(8, 3), (16, 4)])
+def test_simple_host_bands_equal_resident(ctx, bits, c):
+    """Darker / lighter over host frames in row bands equal the resident composite, with crops,
+    a frame selection and a fade."""
+    n, h, w = 30, 21, 33
+    x = make_stack(1600 + bits + c, n, h, w, c, bits=bits)
+    rng = np.random.default_rng(6)
+    crop = (rng.integers(0, 4, (n, 2)).astype(np.int32), w - 3, h - 3)
+    st = cpa.Stack.from_array(ctx, x)
+    row = w * c * (bits // 8)
+    fade = cpa.Fade(False, True, [(0, 0.2), (12, 1.0), (25, 0.5)])
+    for darker in (True, False):
+        proc = cpa.SimpleProcessor(fade=fade, darker=darker)
+        ref = proc.process(st, crop=crop)
+        for band_rows in (1, 5, 64):
+            got = proc.process_host(ctx, x, crop=crop, band_bytes=(band_rows + 3) * row * n + 256)
+            assert np.array_equal(got, ref), (bits, c, darker, band_rows)
+        idx = list(range(2, 28, 3))
+        ref = proc.process(st, image_indices=idx)
+        got = proc.process_host(ctx, x, image_indices=idx, band_bytes=4 * row * len(idx) + 256)
+        assert np.array_equal(got, ref), (bits, c, darker, "frames")
+    st.close()
+
+
 MODES = list(itertools.product([0, 1, 2, 3], [0, 1, 2, 3, 4, 5]))
 
 
