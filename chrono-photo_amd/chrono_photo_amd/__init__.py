@@ -122,7 +122,8 @@ class cp_request(C.Structure):
 
 class cp_server_stats(C.Structure):
     _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("rejected", C.c_uint64),
-                ("pixel_samples", C.c_uint64), ("peak_bytes", C.c_uint64), ("peak_inflight", C.c_uint32)]
+                ("pixel_samples", C.c_uint64), ("peak_bytes", C.c_uint64), ("peak_inflight", C.c_uint32),
+                ("kernel_downloads", C.c_uint32)]
 
 
 class cs_scene(C.Structure):

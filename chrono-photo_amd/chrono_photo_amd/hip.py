@@ -105,11 +105,11 @@ def copy_rows_to_host(out, dev_ptr: int, frame_bytes: int, n_frames: int):
 class PinnedArray:
     """numpy view of page-locked host memory (hipHostMalloc), for asynchronous copies."""
 
-    def __init__(self, shape, dtype):
+    def __init__(self, shape, dtype, flags: int = 0):
         import numpy as np
         nbytes = int(np.prod(shape)) * np.dtype(dtype).itemsize
         p = C.c_void_p()
-        check(hip().hipHostMalloc(C.byref(p), max(nbytes, 1), 0), "hipHostMalloc")
+        check(hip().hipHostMalloc(C.byref(p), max(nbytes, 1), flags), "hipHostMalloc")
         self.ptr = p.value
         buf = (C.c_uint8 * nbytes).from_address(self.ptr)
         self.array = np.frombuffer(buf, dtype=dtype).reshape(shape)

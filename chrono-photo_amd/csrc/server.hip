@@ -1,12 +1,15 @@
 // server.hip -- continuous-batching request loop (cp_server_*).
 //
-// A worker thread owns max_inflight stream slots.  Requests queue in arrival order; the
-// worker admits the queue head whenever a slot is free and the server's HBM budget covers
-// the request's device memory (its uploaded frames and server-side outputs), starts it on
-// the slot's stream (stream-ordered allocation, upload, composite, download, completion
-// event) and retires finished requests by polling their events.  Admitted requests run
-// concurrently on different streams; each request's arithmetic is exactly that of a plain
-// call, so results do not depend on what else is in flight.
+// A worker thread owns max_inflight stream slots and one upload stream.  Requests queue in
+// arrival order; the worker admits the queue head whenever the server's HBM budget covers
+// its device memory (uploaded frames and server-side outputs).  A request that brings host
+// frames first uploads them on the upload stream; the worker watches that upload's event
+// and only then hands the request to a slot, so no compute queue ever holds a wait on a
+// copy (a slot stream waiting on an upload would also stall the other streams that share
+// its hardware queue).  On its slot a request runs composite, download and a completion
+// event; finished requests are retired by polling.  Requests run concurrently on different
+// streams; each request's arithmetic is exactly that of a plain call, so results do not
+// depend on what else is in flight.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -33,6 +36,7 @@ struct Request {
     int status = 1;  // 1: pending, 0: done, < 0: error
     std::string error;
     cp_stack own_stack;
+    hipEvent_t uploaded = nullptr;  // host frames on the device (requests without a stack)
     void* own_frames = nullptr;
     void* own_px = nullptr;
     void* own_mask = nullptr;
@@ -45,6 +49,44 @@ struct Slot {
     ParamArena arena;
     Request* cur = nullptr;
 };
+
+// Copy by a compute queue between device memory and pinned, device-mapped host memory:
+// 16-byte accesses when both ends allow them.
+__global__ void __launch_bounds__(256) copy_kernel(const uint8_t* __restrict__ src, uint8_t* __restrict__ dst,
+                                                   uint64_t bytes) {
+    const uint64_t stride = (uint64_t)gridDim.x * 256;
+    const uint64_t i0 = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    const bool wide = (((uintptr_t)src | (uintptr_t)dst) & 15u) == 0;
+    const uint64_t n16 = wide ? bytes / 16 : 0;
+    for (uint64_t i = i0; i < n16; i += stride)
+        reinterpret_cast<uint4*>(dst)[i] = reinterpret_cast<const uint4*>(src)[i];
+    for (uint64_t i = n16 * 16 + i0; i < bytes; i += stride) dst[i] = src[i];
+}
+
+// The device address of pinned host memory (hipHostMalloc), else null.
+void* mapped_host(void* p) {
+    hipPointerAttribute_t a;
+    if (!p || hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    if (a.type != hipMemoryTypeHost || !a.devicePointer || !a.hostPointer) return nullptr;
+    return (uint8_t*)a.devicePointer + ((uint8_t*)p - (uint8_t*)a.hostPointer);
+}
+
+// Result download: through the DMA path, or, while uploads are in flight (a download
+// queued behind a multi-megabyte upload in a DMA engine would hold its slot), by a copy
+// kernel on the slot's own queue when the host buffer is pinned.
+int download(void* host, const void* dev, uint64_t bytes, hipStream_t s, bool by_kernel, uint32_t* kernel_count) {
+    void* d = by_kernel ? mapped_host(host) : nullptr;
+    if (d) {
+        const uint32_t blocks = (uint32_t)std::min<uint64_t>(1024, (bytes + 4095) / 4096 + 1);
+        hipLaunchKernelGGL(copy_kernel, dim3(blocks), dim3(256), 0, s, (const uint8_t*)dev, (uint8_t*)d, bytes);
+        ++*kernel_count;
+        return hipGetLastError() == hipSuccess ? CP_OK : -1;
+    }
+    return hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, s) == hipSuccess ? CP_OK : -1;
+}
 
 uint64_t request_bytes(const cp_request& r, uint32_t* out_w, uint32_t* out_h) {
     const cp_stack* st = r.job.stack;
@@ -65,6 +107,8 @@ struct cp_server {
     cp_ctx* ctx = nullptr;
     uint64_t budget = 0;
     std::vector<Slot> slots;
+    hipStream_t up_stream = nullptr;
+    std::deque<Request*> uploading, ready;  // admitted: frames in transit / waiting for a slot
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::deque<Request*> queue;
@@ -74,22 +118,33 @@ struct cp_server {
     std::thread worker;
     cp_server_stats stats{};
 
+    int upload(Request* r);
     int start(Request* r, Slot& slot);
     void retire(Slot& slot, int status);
+    void finish(Request* r, int status);
     void loop();
 };
+
+// Allocates a request's frames and starts their upload on the upload stream.
+int cp_server::upload(Request* r) {
+    const cp_request& q = r->req;
+    const uint64_t bytes = (uint64_t)q.width * q.height * q.channels * q.bytes_per_sample * q.n_frames;
+    if (hipMallocAsync(&r->own_frames, bytes, up_stream) != hipSuccess)
+        return fail(CP_ERR_HIP, "server: frame allocation of %llu bytes failed", (unsigned long long)bytes);
+    if (hipMemcpyAsync(r->own_frames, q.host_frames, bytes, hipMemcpyHostToDevice, up_stream) != hipSuccess)
+        return fail(CP_ERR_HIP, "server: frame upload failed");
+    if (hipEventCreateWithFlags(&r->uploaded, hipEventDisableTiming) != hipSuccess ||
+        hipEventRecord(r->uploaded, up_stream) != hipSuccess)
+        return fail(CP_ERR_HIP, "server: upload event failed");
+    return CP_OK;
+}
 
 int cp_server::start(Request* r, Slot& slot) {
     hipStream_t s = slot.stream;
     cp_request& q = r->req;
     cp_job& j = q.job;
     const cp_stack* st = j.stack;
-    if (!st) {
-        const uint64_t bytes = (uint64_t)q.width * q.height * q.channels * q.bytes_per_sample * q.n_frames;
-        if (hipMallocAsync(&r->own_frames, bytes, s) != hipSuccess)
-            return fail(CP_ERR_HIP, "server: frame allocation of %llu bytes failed", (unsigned long long)bytes);
-        if (hipMemcpyAsync(r->own_frames, q.host_frames, bytes, hipMemcpyHostToDevice, s) != hipSuccess)
-            return fail(CP_ERR_HIP, "server: frame upload failed");
+    if (!st) {  // frames uploaded by upload(); the worker saw its event complete
         cp_stack& o = r->own_stack;
         o.ctx = ctx;
         o.width = q.width;
@@ -160,10 +215,11 @@ int cp_server::start(Request* r, Slot& slot) {
         if ((rc = launch_simple(ctx, st, k, s))) return rc;
         r->pixel_samples = (uint64_t)k.n * k.out_w * k.out_h;
     }
-    if (q.host_out_px && hipMemcpyAsync(q.host_out_px, px, px_bytes, hipMemcpyDeviceToHost, s) != hipSuccess)
+    const bool by_kernel = !uploading.empty();
+    if (q.host_out_px && download(q.host_out_px, px, px_bytes, s, by_kernel, &stats.kernel_downloads) != CP_OK)
         return fail(CP_ERR_HIP, "server: result download failed");
     if (q.host_out_mask && mask &&
-        hipMemcpyAsync(q.host_out_mask, mask, mask_bytes, hipMemcpyDeviceToHost, s) != hipSuccess)
+        download(q.host_out_mask, mask, mask_bytes, s, by_kernel, &stats.kernel_downloads) != CP_OK)
         return fail(CP_ERR_HIP, "server: mask download failed");
     if (hipEventRecord(slot.done, s) != hipSuccess) return fail(CP_ERR_HIP, "server: event record failed");
     return CP_OK;
@@ -176,6 +232,21 @@ void cp_server::retire(Slot& slot, int status) {
     if (r->own_px) (void)hipFreeAsync(r->own_px, s);
     if (r->own_mask) (void)hipFreeAsync(r->own_mask, s);
     r->own_frames = r->own_px = r->own_mask = nullptr;
+    slot.cur = nullptr;
+    finish(r, status);
+}
+
+// Releases what is left of a request (not on a slot) and completes it.
+void cp_server::finish(Request* r, int status) {
+    if (r->own_frames) {  // an upload that failed or never reached a slot
+        (void)hipFreeAsync(r->own_frames, up_stream);
+        (void)hipStreamSynchronize(up_stream);
+        r->own_frames = nullptr;
+    }
+    if (r->uploaded) {
+        (void)hipEventDestroy(r->uploaded);
+        r->uploaded = nullptr;
+    }
     used -= r->need;
     r->t_done = Clock::now();
     r->status = status;
@@ -183,7 +254,6 @@ void cp_server::retire(Slot& slot, int status) {
         stats.completed++;
         stats.pixel_samples += r->pixel_samples;
     }
-    slot.cur = nullptr;
 }
 
 void cp_server::loop() {
@@ -205,11 +275,25 @@ void cp_server::loop() {
                 busy = true;
             }
         }
+        // uploads whose frames have arrived become ready for a slot
+        for (auto it = uploading.begin(); it != uploading.end();) {
+            Request* r = *it;
+            hipError_t e = hipEventQuery(r->uploaded);
+            if (e == hipSuccess) {
+                ready.push_back(r);
+                it = uploading.erase(it);
+            } else if (e != hipErrorNotReady) {
+                r->error = hipGetErrorString(e);
+                it = uploading.erase(it);
+                finish(r, CP_ERR_HIP);
+                retired = true;
+            } else {
+                busy = true;
+                ++it;
+            }
+        }
+        // admission in arrival order while the budget allows
         while (!queue.empty()) {
-            Slot* free_slot = nullptr;
-            for (auto& sl : slots)
-                if (!sl.cur) { free_slot = &sl; break; }
-            if (!free_slot) break;
             Request* r = queue.front();
             if (budget && r->need > budget) {
                 queue.pop_front();
@@ -221,9 +305,33 @@ void cp_server::loop() {
                 continue;
             }
             if (budget && used + r->need > budget) break;  // wait for memory to free up
+            // at most one upload in transit per slot: without a budget this bounds the memory
+            if (!r->req.job.stack && uploading.size() >= slots.size()) break;
             queue.pop_front();
             used += r->need;
             if (used > stats.peak_bytes) stats.peak_bytes = used;
+            if (r->req.job.stack) {
+                ready.push_back(r);
+                continue;
+            }
+            int rc = upload(r);
+            if (rc != CP_OK) {
+                r->error = cp_last_error();
+                finish(r, rc);
+                retired = true;
+                continue;
+            }
+            uploading.push_back(r);
+            busy = true;
+        }
+        // ready requests onto free slots
+        while (!ready.empty()) {
+            Slot* free_slot = nullptr;
+            for (auto& sl : slots)
+                if (!sl.cur) { free_slot = &sl; break; }
+            if (!free_slot) break;
+            Request* r = ready.front();
+            ready.pop_front();
             free_slot->cur = r;
             int rc = start(r, *free_slot);
             if (rc != CP_OK) {
@@ -238,8 +346,9 @@ void cp_server::loop() {
             for (auto& sl : slots) inflight += sl.cur != nullptr;
             if (inflight > stats.peak_inflight) stats.peak_inflight = inflight;
         }
+        if (!ready.empty()) busy = true;
         if (retired) cv_done.notify_all();
-        if (stop && queue.empty() && !busy) break;
+        if (stop && queue.empty() && uploading.empty() && ready.empty() && !busy) break;
         if (busy) cv_work.wait_for(lock, std::chrono::microseconds(20));
         else cv_work.wait(lock, [&] { return stop || !queue.empty(); });
     }
@@ -255,6 +364,8 @@ extern "C" CP_API int cp_server_create(cp_ctx* ctx, uint64_t hbm_budget_bytes, u
     srv->budget = hbm_budget_bytes;
     srv->slots.resize(max_inflight);
     (void)hipSetDevice(ctx->device);
+    if (hipStreamCreateWithFlags(&srv->up_stream, hipStreamNonBlocking) != hipSuccess)
+        return fail(CP_ERR_HIP, "server: upload stream creation failed");
     for (auto& sl : srv->slots) {
         if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess ||
             hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess || sl.arena.init(1u << 20) != 0) {
@@ -263,6 +374,7 @@ extern "C" CP_API int cp_server_create(cp_ctx* ctx, uint64_t hbm_budget_bytes, u
                 if (t.done) (void)hipEventDestroy(t.done);
                 t.arena.release();
             }
+            (void)hipStreamDestroy(srv->up_stream);
             return fail(CP_ERR_HIP, "server: stream slot creation failed");
         }
     }
@@ -361,6 +473,8 @@ extern "C" CP_API int cp_server_destroy(cp_server* srv) {
         (void)hipEventDestroy(sl.done);
         sl.arena.release();
     }
+    (void)hipStreamSynchronize(srv->up_stream);
+    (void)hipStreamDestroy(srv->up_stream);
     delete srv;
     return CP_OK;
 }

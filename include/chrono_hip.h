@@ -264,6 +264,8 @@ typedef struct {
     uint64_t pixel_samples;   /* frames * out_h * out_w over completed requests */
     uint64_t peak_bytes;      /* highest admitted HBM use */
     uint32_t peak_inflight;
+    uint32_t kernel_downloads; /* results written to pinned host memory by a copy kernel
+                                  (while uploads were in flight) instead of the DMA path */
 } cp_server_stats;
 
 CP_API int cp_server_create(cp_ctx* ctx, uint64_t hbm_budget_bytes, uint32_t max_inflight,

@@ -66,6 +66,48 @@ def test_windows_and_uploads_in_the_mix_equal_alone(ctx):
     assert np.array_equal(expect[[i for i, e in enumerate(expect) if e[0] == "win"][13]][2], oimg)
 
 
+def test_pinned_results_while_uploads_are_in_flight(ctx):
+    """Results written into pinned host buffers (by a copy kernel while uploads are in flight,
+    else by DMA) equal the same composites run alone."""
+    from chrono_photo_amd import hip as H
+    sc = video_scene(n=30, w=320, h=240)
+    video = cpa.Stack.from_array(ctx, sc.generate())
+    wins = cpa.video_windows(cpa.FrameRange(0, 10, 1), cpa.FrameRange(0, 20, 1), sc.n_frames)
+    proc = cpa.OutlierProcessor(cpa.Threshold.abs(0.05, 0.2), cpa.BackgroundMode.FIRST,
+                                cpa.OutlierSelectionMode.EXTREME)
+    big = cpa.Scene(seed=77, width=640, height=480, n_frames=64, n_disks=1, disk_r_min=9, disk_r_max=9).generate()
+    frames = H.PinnedArray(big.shape, np.uint8)
+    frames.array[...] = big
+    srv = cpa.Server(ctx, max_inflight=4)
+    tickets, outs = [], []
+    for k in range(3):  # each upload of 59 MB stays in flight while windows start
+        out = H.PinnedArray((480, 640, 3), np.uint8)
+        job = cpa.Job(cpa.CP_MODE_OUTLIER, None, None, processor=proc, n_frames=64, bytes_per_sample=1)
+        tickets.append(srv.submit(job, host_frames=frames.array, host_out=out.array))
+        outs.append(("img", None, out))
+        for f, idx in wins[7 * k:7 * k + 7]:
+            o = H.PinnedArray((sc.height, sc.width, 3), np.uint8)
+            m = H.PinnedArray((sc.height, sc.width, 3), np.uint8)
+            job = cpa.Job(cpa.CP_MODE_OUTLIER, video, None, processor=proc, image_indices=idx)
+            tickets.append(srv.submit(job, host_out=o.array, host_mask=m.array))
+            outs.append(("win", idx, (o, m)))
+    for t in tickets:
+        srv.wait(t)
+    st = srv.stats()
+    srv.close()
+    print("downloads by copy kernel:", st["kernel_downloads"])
+    st_big = cpa.Stack.from_array(ctx, big)
+    ref_big, _ = proc.process(st_big, with_mask=False)
+    for kind, idx, bufs in outs:
+        if kind == "img":
+            assert np.array_equal(bufs.array, ref_big)
+        else:
+            ref_img, ref_mask = proc.process(video, image_indices=idx)
+            assert np.array_equal(bufs[0].array, ref_img) and np.array_equal(bufs[1].array, ref_mask)
+    st_big.close()
+    video.close()
+
+
 def test_budget_admission_and_rejection(ctx):
     sc = video_scene(n=20, w=64, h=48)
     x = sc.generate()
