@@ -218,7 +218,8 @@ def run_serving(cpa, H, ctx, seed=4, n_jobs=24, window=60, n_video=600):
             "job_p99_ms": float(np.percentile(jl, 99)) if len(jl) else None,
             "peak_inflight": st["peak_inflight"], "peak_hbm_bytes": st["peak_bytes"],
             "plain": "one request at a time (cp_outlier, image copied to the host)",
-            "served": "cp_server: 8 stream slots, 16 GiB budget, continuous admission"}
+            "served": "cp_server: 8 stream slots, 16 GiB budget, continuous admission, uploads on their own stream",
+            "kernel_downloads": st["kernel_downloads"]}
 
 
 def main():
