@@ -1,11 +1,14 @@
 // k_outlier_fast.h -- register-resident outlier composite (the hot path).
 //
-// Layout.  A workgroup of 256 lanes covers P = 256/G consecutive output pixels of one row.
-// The G lanes of a pixel (adjacent lanes, one DPP quad or half-row) each hold a contiguous
-// block of F frames x C channels in VGPRs, two samples per VGPR (16-bit halves, frames
-// 2j and 2j+1), so the stack tile is read from HBM exactly once and stays on chip for the
-// order-statistic pass and the classification pass.  Frame t's element offset (stack slot
-// + crop origin) comes from a per-workgroup LDS table padded to G*F entries.
+// Layout.  A workgroup of 256 lanes covers P = X*256/G consecutive output pixels of one
+// row.  The G lanes of a pixel group (adjacent lanes, one DPP quad or half-row) each hold a
+// contiguous block of F frames of the group's X pixels x C channels in VGPRs, two samples
+// per VGPR (16-bit halves, frames 2j and 2j+1), so the stack tile is read from HBM exactly
+// once and stays on chip for the order-statistic pass and the classification pass.  With
+// X = 2 (16-bit stacks) each frame of a lane's block is one 16-byte load, which is what lets
+// the lanes' loads run at HBM speed; one pixel per lane needs two or three narrow loads per
+// frame and caps the read rate well below it.  Frame t's element offset (stack slot + crop
+// origin) comes from a per-workgroup LDS table padded to G*F entries.
 //
 // Order statistics (src/chrono.rs:228-255 sorts each weighted channel; any exact selection
 // gives the same values).  The r-th smallest value is found MSB-first: keep the largest
@@ -16,14 +19,15 @@
 // the independent chains interleave.  8-bit stacks need one 8-step level; 16-bit stacks run
 // one level on the high bytes and one on the low bytes clamped into the winning high-byte
 // bucket (values below map to 0, above to 255, which leaves every count #{y < v}, v in
-// [1,255], exact).  The neighbour order statistic of an even-sized median (and the quartile
+// [1,255], exact); their medians first try a single level inside a 256-wide window around
+// an estimate and fall back to the two levels when the result lands on the window's edge.  The neighbour order statistic of an even-sized median (and the quartile
 // interpolation partners) is either the same value (when #{x <= v} covers it) or min{x > v}
 // (packed u16 min).  Frames beyond N are padded with the maximum sample value, which never
 // changes the ranks below N.
 //
 // Classification (src/chrono.rs:257-288) runs over the lane's frames with the reference's
 // f32 operation order; outliers become bits of a per-lane mask.  Each lane then leaves its
-// mask and its largest outlier distance in LDS, and after one barrier P lanes finish one
+// mask and its largest outlier distance in LDS, and after one barrier the lanes finish one
 // pixel each (background and outlier selection / blending, src/chrono.rs:290-493), so the
 // per-pixel tail runs on full waves instead of one lane in G.  The finish re-reads from L2
 // only the few samples an order-dependent blend needs.
