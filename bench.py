@@ -63,6 +63,11 @@ WORKLOADS = {
                    base_lo=40, base_hi=200, sigma=3.0, n_disks=3, disk_r_min=10, disk_r_max=40,
                    disk_val_lo=15, disk_val_hi=25, blob_permille=100, blob_r=30, blob_val=230),
         shake=None, mode="darker"),
+    "lighter_1080p_rgb8_64f": dict(
+        scene=dict(seed=1, width=1920, height=1080, channels=3, bytes_per_sample=1, n_frames=64,
+                   base_lo=40, base_hi=200, sigma=3.0, n_disks=3, disk_r_min=10, disk_r_max=40,
+                   disk_val_lo=15, disk_val_hi=25, blob_permille=100, blob_r=30, blob_val=230),
+        shake=None, mode="lighter"),
     "outlier_640x480_rgb8_16f": dict(
         scene=dict(seed=0, width=640, height=480, channels=3, bytes_per_sample=1, n_frames=16,
                    base_lo=40, base_hi=200, sigma=3.0, n_disks=1, disk_r_min=20, disk_r_max=20,

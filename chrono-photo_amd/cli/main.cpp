@@ -92,11 +92,15 @@ uint64_t parse_bytes(const std::string& s) {
         mul = u == 'K' ? 1ull << 10 : u == 'M' ? 1ull << 20 : 1ull << 30;
         num = s.substr(0, s.size() - 1);
     }
+    size_t used = 0;
+    uint64_t v = 0;
     try {
-        return (uint64_t)std::stoull(num) * mul;
+        v = (uint64_t)std::stoull(num, &used);
     } catch (...) {
-        die("--hbm-budget expects a size such as 512M or 8G, got " + s);
+        used = 0;
     }
+    if (used == 0 || used != num.size()) die("--hbm-budget expects a size such as 512M or 8G, got " + s);
+    return v * mul;
 }
 
 // The reference's option-file tokenizer (src/cli.rs:245-266): quotes group, spaces split.

@@ -170,6 +170,8 @@ def test_cli_option_errors(tmp_path):
     assert r.returncode != 0 and "unknown option" in r.stderr
     r = _cli("--pattern", str(tmp_path / "none*.png"), "--output", "o.png")
     assert r.returncode != 0 and "No files found" in r.stderr
+    r = _cli("--pattern", "x*.png", "--output", "o.png", "--hbm-budget", "12X")
+    assert r.returncode != 0 and "--hbm-budget" in r.stderr
 
 
 # ------------------------------------------------------------------ GPU: end to end
