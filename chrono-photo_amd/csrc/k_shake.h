@@ -6,19 +6,26 @@
 // (ox, oy) (calc_diffs, :345-385).  The offset with the smallest sum wins; ties go to the
 // first index in row-major (oy, ox) order (min_by_key, :294-297).
 //
-// u8 sums wrap in i32 like the reference (release build, overflow-checks = false); they
-// are accumulated in u32, which is the same value modulo 2^32.  u16 stacks accumulate in
-// int64.
+// u8 sums wrap in i32 like the reference (release build, overflow-checks = false): the
+// kernels sum exactly and the arg-min reads each total modulo 2^32, which is that wrapped
+// value.  u16 sums are exact 64-bit integers.
 //
-// One workgroup per (frame, anchor) pair stages the anchor's search region of frame f in
-// LDS; each lane owns candidate offsets and streams the frame-0 window from cache
-// (wave-uniform addresses).  Per-anchor partial sums go to a per-frame table with 64-bit
-// atomics (integer addition commutes, so the result does not depend on order); a second
-// kernel takes the arg-min.
+// Work split.  A workgroup owns one (frame, anchor, chunk of window rows): it stages the
+// frame-f search region those rows reach (chunk + 2s rows) and the frame-0 window rows in
+// LDS.  A lane owns a strip of K horizontally adjacent candidate offsets (ox0..ox0+K-1, oy)
+// over a sub-chunk of the rows; the strip's K region pixels slide through registers along
+// dx, so each region sample read from LDS serves K terms and each window sample (the same
+// address for the whole wave) serves K terms as well.  Terms are summed in f64: (w - v)^2
+// of two integers below 2^16 is exact, and so is every partial sum below 2^53, which the
+// host guarantees by bounding the rows per lane.  Each lane adds its K exact sums to the
+// per-frame table with 64-bit atomics (integer addition commutes, so the result does not
+// depend on order); a second kernel takes the arg-min.
 #pragma once
 #include "device_common.h"
 
 namespace cp {
+
+constexpr int kShakeStrip = 5;  // candidate offsets per lane (K)
 
 struct ShakeK {
     const uint8_t* stack;
@@ -28,50 +35,124 @@ struct ShakeK {
     const int32_t* anchors;   // device, 2 per anchor
     uint32_t n_anchors;
     int32_t r, s;
+    int32_t chunk_rows;       // window rows per workgroup (blockIdx.z chunks them)
+    int32_t nsub;             // row sub-chunks per workgroup (one per group of strip lanes)
+    int32_t spr;              // strips per offset row: ceil((2s+1) / K)
+    int32_t strips_pad;       // spr * (2s+1) rounded up to whole waves
+    int32_t pitch_px;         // LDS region row pitch in pixels: spr * K + 2r
     unsigned long long* sums; // [n][(2s+1)^2]
     int32_t* offsets;         // [n][2]
 };
 
+// LDS bytes of one workgroup for a chunk of `rows` window rows.
+inline size_t shake_lds_bytes(int32_t rows, int32_t r, int32_t s, int32_t pitch_px, int32_t channels) {
+    const size_t srows = (size_t)rows + 2 * s;
+    return (srows * pitch_px + (size_t)rows * (2 * r + 1)) * channels * sizeof(uint16_t);
+}
+
+// One dx step of a strip: load region pixel dx+K-1 into its register slot, then add the
+// K terms (w(dx) - v(dx + j))^2 for every channel.  U = dx mod K names the slots statically.
+template <int C, int K, int U>
+__device__ __forceinline__ void shake_step(const uint16_t* rr, const uint16_t* wr, int32_t dx, double (&v)[K][C],
+                                           double (&acc)[K]) {
+#pragma unroll
+    for (int c = 0; c < C; ++c) v[(U + K - 1) % K][c] = (double)(uint32_t)rr[(dx + K - 1) * C + c];
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        const double w = (double)(uint32_t)wr[dx * C + c];
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            const double d = w - v[(U + j) % K][c];
+            acc[j] = __fma_rn(d, d, acc[j]);
+        }
+    }
+}
+
+template <int C, int K, int U>
+__device__ __forceinline__ void shake_steps(const uint16_t* rr, const uint16_t* wr, int32_t dx, double (&v)[K][C],
+                                            double (&acc)[K]) {
+    if constexpr (U < K) {
+        shake_step<C, K, U>(rr, wr, dx + U, v, acc);
+        shake_steps<C, K, U + 1>(rr, wr, dx, v, acc);
+    }
+}
+
+// Remainder of a row (fewer than K steps, count uniform across the wave).
+template <int C, int K, int U>
+__device__ __forceinline__ void shake_tail(const uint16_t* rr, const uint16_t* wr, int32_t dx, int32_t left,
+                                           double (&v)[K][C], double (&acc)[K]) {
+    if constexpr (U < K - 1) {
+        if (U < left) {
+            shake_step<C, K, U>(rr, wr, dx + U, v, acc);
+            shake_tail<C, K, U + 1>(rr, wr, dx, left, v, acc);
+        }
+    }
+}
+
 template <int B, int C>
 __global__ void __launch_bounds__(256) shake_ssd_kernel(ShakeK k) {
-    extern __shared__ uint16_t region[];  // (2r+2s+1)^2 * C samples of frame f
+    constexpr int K = kShakeStrip;
+    extern __shared__ uint16_t lds[];
     const uint32_t f = blockIdx.x + 1;
     const uint32_t a = blockIdx.y;
     const int32_t r = k.r, s = k.s;
-    const int32_t size = 2 * r + 1, ssize = 2 * s + 1, rsize = size + 2 * s;
+    const int32_t size = 2 * r + 1, ssize = 2 * s + 1;
+    const int32_t c0 = (int32_t)blockIdx.z * k.chunk_rows;
+    const int32_t crows = min(size, c0 + k.chunk_rows) - c0;
+    const int32_t srows = crows + ssize - 1;  // region rows: offset rows 0..2s over the chunk
+    const int32_t pitch = k.pitch_px * C;     // samples per LDS region row
+    const int32_t wl = size * C;              // samples per window row
+    uint16_t* region = lds;
+    uint16_t* win = lds + srows * pitch;
     const int32_t cx = k.anchors[2 * a], cy = k.anchors[2 * a + 1];
-    // region origin: (cx - s - r, cy - s - r)
-    const int32_t rx0 = cx - s - r, ry0 = cy - s - r;
-    const uint64_t fbase = k.pos_off[f];
-    const uint64_t wbase = k.pos_off[0];
-    const int32_t rlen = rsize * rsize * C;
-    for (int32_t i = threadIdx.x; i < rlen; i += blockDim.x) {
-        int32_t ch = i % C, p = i / C;
-        int32_t xx = rx0 + p % rsize, yy = ry0 + p / rsize;
-        region[i] = (uint16_t)load_sample<B>(k.stack, fbase + ((uint64_t)yy * k.stack_w + xx) * C + ch);
+    // Region pixel (x, y) is frame-f pixel (cx - s - r + x, cy - s - r + c0 + y); the
+    // pixels past 2r + 2s only pad the last strip of a row and are never summed.
+    {
+        const int32_t rlen = (2 * r + 2 * s + 1) * C;
+        const uint64_t fbase = k.pos_off[f] + ((uint64_t)(cy - s - r + c0) * k.stack_w + (cx - s - r)) * C;
+#pragma unroll 2
+        for (int32_t y = 0; y < srows; ++y) {
+            const uint64_t row = fbase + (uint64_t)y * k.stack_w * C;
+            for (int32_t e = threadIdx.x; e < pitch; e += blockDim.x)
+                region[y * pitch + e] = e < rlen ? (uint16_t)load_sample<B>(k.stack, row + e) : (uint16_t)0;
+        }
+        const uint64_t wbase = k.pos_off[0] + ((uint64_t)(cy - r + c0) * k.stack_w + (cx - r)) * C;
+#pragma unroll 2
+        for (int32_t y = 0; y < crows; ++y) {
+            const uint64_t row = wbase + (uint64_t)y * k.stack_w * C;
+            for (int32_t e = threadIdx.x; e < wl; e += blockDim.x)
+                win[y * wl + e] = (uint16_t)load_sample<B>(k.stack, row + e);
+        }
     }
     __syncthreads();
-    const int32_t noff = ssize * ssize;
-    for (int32_t o = threadIdx.x; o < noff; o += blockDim.x) {
-        const int32_t ox = o % ssize, oy = o / ssize;
-        uint64_t acc64 = 0;
-        uint32_t acc32 = 0;
-        for (int32_t dy = 0; dy < size; ++dy) {
-            const uint64_t wrow = wbase + ((uint64_t)(cy + dy - r) * k.stack_w + (cx - r)) * C;
-            const uint16_t* srow = region + ((oy + dy) * rsize + ox) * C;
-            for (int32_t dx = 0; dx < size; ++dx) {
+    const int32_t nstrips = k.spr * ssize;
+    const int32_t ntasks = k.strips_pad * k.nsub;
+    const uint64_t noff = (uint64_t)ssize * ssize;
+    for (int32_t t = threadIdx.x; t < ntasks; t += blockDim.x) {
+        // strips_pad is a multiple of 64, so a wave shares its sub-chunk (and window reads)
+        const int32_t sub = t / k.strips_pad, strip = t - sub * k.strips_pad;
+        if (strip >= nstrips) continue;
+        const int32_t oy = strip / k.spr, ox0 = (strip - oy * k.spr) * K;
+        const int32_t lo = sub * crows / k.nsub, hi = (sub + 1) * crows / k.nsub;
+        double acc[K];
 #pragma unroll
-                for (int ch = 0; ch < C; ++ch) {
-                    int32_t w = (int32_t)load_sample<B>(k.stack, wrow + dx * C + ch);
-                    int32_t v = (int32_t)srow[dx * C + ch];
-                    int32_t d = w - v;
-                    if constexpr (B == 1) acc32 += (uint32_t)(d * d);
-                    else acc64 += (uint64_t)((int64_t)d * d);
-                }
-            }
+        for (int j = 0; j < K; ++j) acc[j] = 0.0;
+        for (int32_t y = lo; y < hi; ++y) {
+            const uint16_t* rr = region + (oy + y) * pitch + ox0 * C;
+            const uint16_t* wr = win + y * wl;
+            double v[K][C];
+#pragma unroll
+            for (int u = 0; u < K - 1; ++u)
+#pragma unroll
+                for (int c = 0; c < C; ++c) v[u][c] = (double)(uint32_t)rr[u * C + c];
+            int32_t dx = 0;
+            for (; dx + K <= size; dx += K) shake_steps<C, K, 0>(rr, wr, dx, v, acc);
+            shake_tail<C, K, 0>(rr, wr, dx, size - dx, v, acc);
         }
-        unsigned long long add = B == 1 ? (unsigned long long)acc32 : (unsigned long long)acc64;
-        atomicAdd(&k.sums[(uint64_t)f * noff + o], add);
+        unsigned long long* out = k.sums + (uint64_t)f * noff + (uint64_t)oy * ssize + ox0;
+#pragma unroll
+        for (int j = 0; j < K; ++j)
+            if (ox0 + j < ssize) atomicAdd(out + j, (unsigned long long)acc[j]);
     }
 }
 

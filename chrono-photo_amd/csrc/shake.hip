@@ -63,15 +63,42 @@ extern "C" CP_API int cp_shake(cp_ctx* ctx, const cp_stack* st, const uint32_t* 
     k.sums = (unsigned long long*)sums.ptr;
     k.offsets = (int32_t*)offs.ptr;
     hipError_t e = hipMemsetAsync(sums.ptr, 0, noff * n * 8, hs);
-    const int32_t rsize = (int32_t)(2 * r + 1 + 2 * s);
-    const size_t lds = (size_t)rsize * rsize * st->channels * sizeof(uint16_t);
-    if (e == hipSuccess && lds > 64 * 1024) {
+    // Launch plan: strips of kShakeStrip offsets per lane, whole waves per row sub-chunk,
+    // about 8 window rows per sub-chunk, as many rows per workgroup as LDS holds.
+    constexpr int32_t K = kShakeStrip;
+    const int32_t size = (int32_t)(2 * r + 1), C = (int32_t)st->channels;
+    k.spr = (ssize + K - 1) / K;
+    k.strips_pad = (k.spr * ssize + 63) / 64 * 64;
+    int32_t block = 256;
+    if (k.strips_pad <= 256) {
+        k.nsub = 256 / k.strips_pad;
+        block = k.strips_pad * k.nsub;
+    } else {
+        k.nsub = 1;
+    }
+    k.pitch_px = k.spr * K + (int32_t)(2 * r);
+    // Exact f64 sums: each lane adds ceil(rows / nsub) * size * C terms below maxsq.
+    const double maxsq = st->bps == 2 ? 65535.0 * 65535.0 : 255.0 * 255.0;
+    auto fits = [&](int32_t rows) {
+        const double terms = (double)((rows + k.nsub - 1) / k.nsub) * size * C;
+        return shake_lds_bytes(rows, (int32_t)r, (int32_t)s, k.pitch_px, C) <= 64 * 1024 &&
+               terms * maxsq < 9007199254740992.0;
+    };
+    const int32_t want = std::min<int32_t>(size, 8 * k.nsub);
+    const int32_t even_chunks = (size + want - 1) / want;
+    int32_t rows = (size + even_chunks - 1) / even_chunks;
+    while (rows > 1 && !fits(rows)) --rows;
+    k.chunk_rows = rows;
+    const uint32_t nchunk = (uint32_t)((size + rows - 1) / rows);
+    const size_t lds = shake_lds_bytes(rows, (int32_t)r, (int32_t)s, k.pitch_px, C);
+    if (e == hipSuccess && (!fits(rows) || n_anchors > 65535)) {
         e = hipErrorInvalidValue;
-        rc = fail(CP_ERR_INVALID_ARG, "anchor region of %zu bytes exceeds the 64 KiB LDS window", lds);
+        rc = fail(CP_ERR_INVALID_ARG, "anchor search region needs %zu bytes of LDS per window row (64 KiB at most)",
+                  shake_lds_bytes(1, (int32_t)r, (int32_t)s, k.pitch_px, C));
     }
     if (e == hipSuccess) {
-        dim3 grid(n - 1, n_anchors), block(256);
-        auto launch = [&](auto kern) { hipLaunchKernelGGL(kern, grid, block, lds, hs, k); };
+        dim3 grid(n - 1, n_anchors, nchunk), blk(block);
+        auto launch = [&](auto kern) { hipLaunchKernelGGL(kern, grid, blk, lds, hs, k); };
         const bool wide = st->bps == 2;
         switch (st->channels) {
         case 1: wide ? launch(shake_ssd_kernel<2, 1>) : launch(shake_ssd_kernel<1, 1>); break;

@@ -303,6 +303,43 @@ def test_shake_u8_wrapping_sum_matches(ctx):
     assert np.array_equal(off, O.shake(x, anchors, 30, 6))
 
 
+@pytest.mark.parametrize("c,bits,r,s", [(1, 8, 4, 1), (2, 16, 3, 2), (4, 8, 6, 3), (4, 16, 9, 7),
+                                         (3, 16, 12, 5), (1, 16, 20, 0), (3, 8, 0, 4)])
+def test_shake_unstructured_frames(ctx, c, bits, r, s):
+    """Random frames: every candidate's sum differs, so the arg-min exposes any wrong term.
+    Covers partial offset strips (2s+1 not a multiple of 5), several row chunks and 1-4
+    channels."""
+    rng = np.random.default_rng(100 + c * 7 + r + s)
+    hi = (1 << bits) - 1
+    h, w = 2 * (r + s) + 17, 2 * (r + s) + 23
+    x = rng.integers(0, hi + 1, (6, h, w, c)).astype(np.uint8 if bits == 8 else np.uint16)
+    anchors = [(r + s, r + s), (w - 1 - r - s, h - 1 - r - s), (w // 2, h // 2)]
+    st = cpa.Stack.from_array(ctx, x)
+    off = cpa.ShakeAnalyzer().analyze(st, anchors, r, s)
+    assert np.array_equal(off, O.shake(x, anchors, r, s))
+
+
+def test_shake_large_anchor_region(ctx):
+    """r=50, s=10 on RGB16: a 121x121 search region, split over several workgroups' LDS."""
+    shifts = [(0, 0), (3, -2), (-7, 5), (10, 10), (-10, -9)]
+    x = textured(len(shifts), 150, 170, shifts, bits=16, seed=11)
+    anchors = [(70, 70), (100, 75)]
+    st = cpa.Stack.from_array(ctx, x)
+    off = cpa.ShakeAnalyzer().analyze(st, anchors, 50, 10)
+    assert np.array_equal(off, O.shake(x, anchors, 50, 10))
+    assert [tuple(o) for o in off] == shifts
+
+
+@pytest.mark.parametrize("bits", [8, 16])
+def test_shake_ties_keep_first_offset(ctx, bits):
+    """Constant frames: every candidate sums to 0 and min_by_key keeps offset (-s, -s)."""
+    x = np.full((4, 40, 40, 3), 200, np.uint8 if bits == 8 else np.uint16)
+    st = cpa.Stack.from_array(ctx, x)
+    off = cpa.ShakeAnalyzer().analyze(st, [(20, 20)], 6, 7)
+    assert np.array_equal(off, O.shake(x, [(20, 20)], 6, 7))
+    assert off[1:].tolist() == [[-7, -7]] * 3
+
+
 def test_shake_anchor_errors(ctx):
     x = textured(2, 30, 30, [(0, 0), (1, 1)])
     st = cpa.Stack.from_array(ctx, x)
