@@ -84,18 +84,29 @@ def _stack_from_scene(cpa, ctx, sc):
     return st
 
 
-def run_batching(cpa, H, ctx, seed=3, n_stacks=48):
-    """48 independent RGB8 stacks (frames U{8..200}, 640x480..1920x1080, abs/rel thresholds
-    alternating, the six --outlier modes round-robin): one graph of per-group launches vs one
-    launch per stack."""
+BATCH_SETS = {
+    # stacks that each fill the GPU on their own: batching can only save the gaps between launches
+    "large": dict(n_stacks=48, frames=(8, 200), width=(640, 1920), height=(480, 1080), seed=3),
+    # previews / thumbnails: each plain launch is bound by launch and synchronisation latency
+    "small": dict(n_stacks=256, frames=(8, 64), width=(160, 640), height=(120, 480), seed=5),
+}
+
+
+def run_batching(cpa, H, ctx, kind="large"):
+    """Independent RGB8 stacks of uneven sizes (abs/rel thresholds alternating, the six
+    --outlier modes round-robin): one graph of per-group launches vs one launch per stack.
+    "large": 48 stacks, frames U{8..200}, 640x480..1920x1080; "small": 256 stacks, frames
+    U{8..64}, 160x120..640x480."""
     import numpy as np
-    rng = np.random.default_rng(seed)
+    cfg = BATCH_SETS[kind]
+    n_stacks = cfg["n_stacks"]
+    rng = np.random.default_rng(cfg["seed"])
     jobs, bufs, total = [], [], 0
     sels = list(cpa.OutlierSelectionMode)
     for i in range(n_stacks):
-        n = int(rng.integers(8, 201))
-        w = int(rng.integers(640, 1921))
-        h = int(rng.integers(480, 1081))
+        n = int(rng.integers(cfg["frames"][0], cfg["frames"][1] + 1))
+        w = int(rng.integers(cfg["width"][0], cfg["width"][1] + 1))
+        h = int(rng.integers(cfg["height"][0], cfg["height"][1] + 1))
         sc = cpa.Scene(seed=1000 + i, width=w, height=h, n_frames=n, n_disks=2, disk_r_min=10,
                        disk_r_max=30, blob_permille=100)
         st = _stack_from_scene(cpa, ctx, sc)
@@ -138,10 +149,12 @@ def run_batching(cpa, H, ctx, seed=3, n_stacks=48):
         st.close()
         px.free()
         mask.free()
-    return {"stacks": n_stacks, "pixel_samples": total,
+    return {"stacks": n_stacks, "frames": list(cfg["frames"]),
+            "sizes": f"{cfg['width'][0]}x{cfg['height'][0]}..{cfg['width'][1]}x{cfg['height'][1]}",
+            "pixel_samples": total, "plain_ms": plain_s * 1e3, "batched_ms": batch_s * 1e3,
             "plain_value": total / plain_s, "batched_value": total / batch_s,
             "speedup": plain_s / batch_s, "byte_identical": bool(same),
-            "plain": "48 sequential cp_outlier launches, one stack each, each waited for",
+            "plain": f"{n_stacks} sequential cp_outlier launches, one stack each, each waited for",
             "batched": "one graph replay: one launch per tile-shape group (cp_batch_create/cp_batch_launch)"}
 
 
@@ -378,8 +391,13 @@ def main():
         out_px.free()
         out_mask.free()
         t0 = time.time()
-        extra["batching"] = run_batching(cpa, H, ctx)
-        print(f"[bench] batching: {extra['batching']['speedup']:.2f}x over plain "
+        extra["batching"] = run_batching(cpa, H, ctx, "large")
+        print(f"[bench] batching (48 large stacks): {extra['batching']['speedup']:.2f}x over plain "
+              f"({time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
+        t0 = time.time()
+        extra["batching_small"] = run_batching(cpa, H, ctx, "small")
+        print(f"[bench] batching (256 small stacks): {extra['batching_small']['speedup']:.2f}x over plain, "
+              f"{extra['batching_small']['plain_ms']:.2f} -> {extra['batching_small']['batched_ms']:.2f} ms "
               f"({time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
         t0 = time.time()
         extra["serving"] = run_serving(cpa, H, ctx)
