@@ -53,16 +53,17 @@ extern "C" CP_API int cp_shake(cp_ctx* ctx, const cp_stack* st, const uint32_t* 
     k.r = (int32_t)r;
     k.s = (int32_t)s;
     DeviceBuffer sums, offs;
-    int rc = ctx->charge(noff * n * 8 + n * 8);
+    const uint64_t sums_bytes = (noff * n + 1) * 8;  // per-frame tables + the window constant
+    int rc = ctx->charge(sums_bytes + n * 8);
     if (rc) return rc;
-    if (hipMalloc(&sums.ptr, noff * n * 8) != hipSuccess || hipMalloc(&offs.ptr, n * 8) != hipSuccess) {
+    if (hipMalloc(&sums.ptr, sums_bytes) != hipSuccess || hipMalloc(&offs.ptr, n * 8) != hipSuccess) {
         if (sums.ptr) (void)hipFree(sums.ptr);
-        ctx->uncharge(noff * n * 8 + n * 8);
+        ctx->uncharge(sums_bytes + n * 8);
         return fail(CP_ERR_HIP, "hipMalloc for shake sums failed");
     }
     k.sums = (unsigned long long*)sums.ptr;
     k.offsets = (int32_t*)offs.ptr;
-    hipError_t e = hipMemsetAsync(sums.ptr, 0, noff * n * 8, hs);
+    hipError_t e = hipMemsetAsync(sums.ptr, 0, sums_bytes, hs);
     // Launch plan: strips of kShakeStrip offsets per lane, whole waves per row sub-chunk,
     // about 8 window rows per sub-chunk, as many rows per workgroup as LDS holds.
     constexpr int32_t K = kShakeStrip;
@@ -77,11 +78,11 @@ extern "C" CP_API int cp_shake(cp_ctx* ctx, const cp_stack* st, const uint32_t* 
         k.nsub = 1;
     }
     k.pitch_px = k.spr * K + (int32_t)(2 * r);
-    // Exact f64 sums: each lane adds ceil(rows / nsub) * size * C terms below maxsq.
+    // Exact f64 sums: each lane adds ceil(rows / nsub) * size * C products below maxsq.
     const double maxsq = st->bps == 2 ? 65535.0 * 65535.0 : 255.0 * 255.0;
     auto fits = [&](int32_t rows) {
         const double terms = (double)((rows + k.nsub - 1) / k.nsub) * size * C;
-        return shake_lds_bytes(rows, (int32_t)r, (int32_t)s, k.pitch_px, C) <= 64 * 1024 &&
+        return shake_lds(rows, (int32_t)r, (int32_t)s, k.pitch_px, C).bytes <= 64 * 1024 &&
                terms * maxsq < 9007199254740992.0;
     };
     const int32_t want = std::min<int32_t>(size, 8 * k.nsub);
@@ -90,11 +91,11 @@ extern "C" CP_API int cp_shake(cp_ctx* ctx, const cp_stack* st, const uint32_t* 
     while (rows > 1 && !fits(rows)) --rows;
     k.chunk_rows = rows;
     const uint32_t nchunk = (uint32_t)((size + rows - 1) / rows);
-    const size_t lds = shake_lds_bytes(rows, (int32_t)r, (int32_t)s, k.pitch_px, C);
+    const size_t lds = shake_lds(rows, (int32_t)r, (int32_t)s, k.pitch_px, C).bytes;
     if (e == hipSuccess && (!fits(rows) || n_anchors > 65535)) {
         e = hipErrorInvalidValue;
         rc = fail(CP_ERR_INVALID_ARG, "anchor search region needs %zu bytes of LDS per window row (64 KiB at most)",
-                  shake_lds_bytes(1, (int32_t)r, (int32_t)s, k.pitch_px, C));
+                  (size_t)shake_lds(1, (int32_t)r, (int32_t)s, k.pitch_px, C).bytes);
     }
     if (e == hipSuccess) {
         dim3 grid(n - 1, n_anchors, nchunk), blk(block);
@@ -116,6 +117,6 @@ extern "C" CP_API int cp_shake(cp_ctx* ctx, const cp_stack* st, const uint32_t* 
     (void)hipStreamSynchronize(hs);
     (void)hipFree(sums.ptr);
     (void)hipFree(offs.ptr);
-    ctx->uncharge(noff * n * 8 + n * 8);
+    ctx->uncharge(sums_bytes + n * 8);
     return rc;
 }
