@@ -284,6 +284,29 @@ __device__ __forceinline__ void rank_pairs(const uint32_t (&raw)[C][NW], uint32_
     }
 }
 
+// The three rank pairs of a relative threshold (median, first and third quartile) of an
+// 8-bit stack in one byte search: the bytes are packed once and the 3*C chains interleave.
+template <int G, int NW, int C>
+__device__ __forceinline__ void rank_triples_u8(const uint32_t (&raw)[C][NW], const RankPlan& plan, int32_t ntot,
+                                                uint32_t (&m0)[C], uint32_t (&m1)[C], uint32_t (&a0)[C],
+                                                uint32_t (&a1)[C], uint32_t (&b0)[C], uint32_t (&b1)[C]) {
+    constexpr int ND = NW / 2;
+    const uint32_t kk[3] = {plan.m0, plan.a0, plan.b0};
+    uint32_t y[C][ND], sel[C][3], cle[C][3];
+#pragma unroll
+    for (int c = 0; c < C; ++c) pack_low_bytes<NW>(raw[c], y[c]);
+    byte_select_multi<G, ND, C, 3, true>(y, kk, ntot, sel, cle);
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        m0[c] = sel[c][0];
+        a0[c] = sel[c][1];
+        b0[c] = sel[c][2];
+        m1[c] = plan.m1 == plan.m0 ? m0[c] : neighbour<G, NW>(raw[c], m0[c], cle[c][0], plan.m1);
+        a1[c] = plan.a1 == plan.a0 ? a0[c] : neighbour<G, NW>(raw[c], a0[c], cle[c][1], plan.a1);
+        b1[c] = plan.b1 == plan.b0 ? b0[c] : neighbour<G, NW>(raw[c], b0[c], cle[c][2], plan.b1);
+    }
+}
+
 // src/chrono.rs:257-278 on one frame's C samples (f32, reference operation order).
 template <int C>
 __device__ __forceinline__ float frame_dist(const uint32_t (&s)[4], const float (&med)[4],
@@ -756,10 +779,14 @@ __device__ __forceinline__ void outlier_tile(const OutlierK& k, uint32_t bx, uin
             constexpr int E = decltype(e_t)::value;
             const uint32_t (&re)[C][NW] = *reinterpret_cast<const uint32_t(*)[C][NW]>(&raw[E * C][0]);
             uint32_t m0[C], m1[C], a0[C], a1[C], b0[C], b1[C];
-            rank_pairs<B, G, NW, C, ABS>(re, plan.m0, plan.m1, ntot, m0, m1);
-            if constexpr (!ABS) {
-                rank_pairs<B, G, NW, C, false>(re, plan.a0, plan.a1, ntot, a0, a1);
-                rank_pairs<B, G, NW, C, false>(re, plan.b0, plan.b1, ntot, b0, b1);
+            if constexpr (!ABS && B == 1) {
+                rank_triples_u8<G, NW, C>(re, plan, ntot, m0, m1, a0, a1, b0, b1);
+            } else {
+                rank_pairs<B, G, NW, C, ABS>(re, plan.m0, plan.m1, ntot, m0, m1);
+                if constexpr (!ABS) {
+                    rank_pairs<B, G, NW, C, false>(re, plan.a0, plan.a1, ntot, a0, a1);
+                    rank_pairs<B, G, NW, C, false>(re, plan.b0, plan.b1, ntot, b0, b1);
+                }
             }
 #pragma unroll
             for (int c = 0; c < 4; ++c) {

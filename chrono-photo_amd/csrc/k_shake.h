@@ -10,9 +10,9 @@
 // kernels sum exactly and the arg-min reads each total modulo 2^32, which is that wrapped
 // value.  u16 sums are exact 64-bit integers.
 //
-// Work split.  A workgroup owns one (frame, anchor, chunk of window rows): it stages the
-// frame-f search region those rows reach (chunk + 2s rows, u16) and the frame-0 window rows
-// (f32) in LDS.  Each candidate's sum is expanded as
+// Work split.  A workgroup owns one (frame, anchor, chunk of candidate rows, chunk of window
+// rows): it stages the frame-f search region those rows reach (window chunk + candidate
+// chunk - 1 rows, u16) and the frame-0 window rows (f32) in LDS.  Each candidate's sum is expanded as
 //   sum (w - v)^2 = sum w^2 + sum v^2 - 2 sum w*v,
 // all three exact integers.  sum w^2 is one constant for every frame and candidate: the
 // frame-1 workgroups add it to a spare slot of the table, and the arg-min kernel adds it back.
@@ -41,22 +41,26 @@ struct ShakeK {
     uint32_t n_anchors;
     int32_t r, s;
     int32_t chunk_rows;       // window rows per workgroup (blockIdx.z chunks them)
+    int32_t cand_rows;        // candidate rows oy per workgroup
+    int32_t cand_chunks;      // ceil((2s+1) / cand_rows); blockIdx.y = anchor * cand_chunks + chunk
     int32_t nsub;             // row sub-chunks per workgroup (one per group of strip lanes)
     int32_t spr;              // strips per offset row: ceil((2s+1) / K)
-    int32_t strips_pad;       // spr * (2s+1) rounded up to whole waves
+    int32_t strips_pad;       // spr * cand_rows rounded up to whole waves
     int32_t pitch_px;         // LDS region row pitch in pixels: spr * K + 2r
     unsigned long long* sums; // [n][(2s+1)^2], then the sum of squared window samples
     int32_t* offsets;         // [n][2]
 };
 
-// LDS layout of one workgroup for a chunk of `rows` window rows: region samples (u16,
-// rows + 2s region rows of pitch_px pixels), window samples (f32, rows x (2r+1) pixels),
-// squared-sample box sums (f64, rows + 2s region rows x (2s+1) candidate columns).
+// LDS layout of one workgroup for `rows` window rows and `crow` candidate rows: region
+// samples (u16, rows + crow - 1 region rows of pitch_px pixels), window samples (f32,
+// rows x (2r+1) pixels), squared-sample box sums (f64, one per region row and candidate
+// column).
 struct ShakeLds {
     uint32_t win_off, rq_off, bytes;
 };
-__host__ __device__ inline ShakeLds shake_lds(int32_t rows, int32_t r, int32_t s, int32_t pitch_px, int32_t channels) {
-    const uint64_t srows = (uint64_t)rows + 2 * s;
+__host__ __device__ inline ShakeLds shake_lds(int32_t rows, int32_t crow, int32_t r, int32_t s, int32_t pitch_px,
+                                             int32_t channels) {
+    const uint64_t srows = (uint64_t)rows + crow - 1;
     const uint64_t region = srows * pitch_px * channels * 2;
     const uint64_t win_off = (region + 7) / 8 * 8;
     const uint64_t rq_off = (win_off + (uint64_t)rows * (2 * r + 1) * channels * 4 + 7) / 8 * 8;
@@ -107,21 +111,24 @@ __global__ void __launch_bounds__(256) shake_ssd_kernel(ShakeK k) {
     constexpr int K = kShakeStrip;
     extern __shared__ double lds_f64[];
     const uint32_t f = blockIdx.x + 1;
-    const uint32_t a = blockIdx.y;
+    const uint32_t a = blockIdx.y / (uint32_t)k.cand_chunks;
+    const int32_t oc = (int32_t)(blockIdx.y - a * (uint32_t)k.cand_chunks);
     const int32_t r = k.r, s = k.s;
     const int32_t size = 2 * r + 1, ssize = 2 * s + 1;
     const int32_t c0 = (int32_t)blockIdx.z * k.chunk_rows;
     const int32_t crows = min(size, c0 + k.chunk_rows) - c0;
-    const int32_t srows = crows + ssize - 1;  // region rows: offset rows 0..2s over the chunk
+    const int32_t oy0 = oc * k.cand_rows;
+    const int32_t orows = min(ssize, oy0 + k.cand_rows) - oy0;
+    const int32_t srows = crows + orows - 1;  // region rows: the candidate rows over the chunk
     const int32_t pitch = k.pitch_px * C;     // samples per LDS region row
     const int32_t wl = size * C;              // samples per window row
-    const ShakeLds lay = shake_lds(crows, r, s, k.pitch_px, C);
+    const ShakeLds lay = shake_lds(crows, orows, r, s, k.pitch_px, C);
     uint8_t* lds = reinterpret_cast<uint8_t*>(lds_f64);
     uint16_t* region = reinterpret_cast<uint16_t*>(lds);
     float* win = reinterpret_cast<float*>(lds + lay.win_off);
     double* rq = reinterpret_cast<double*>(lds + lay.rq_off);
     const int32_t cx = k.anchors[2 * a], cy = k.anchors[2 * a + 1];
-    // Region pixel (x, y) is frame-f pixel (cx - s - r + x, cy - s - r + c0 + y); the
+    // Region pixel (x, y) is frame-f pixel (cx - s - r + x, cy - s - r + oy0 + c0 + y); the
     // pixels past 2r + 2s only pad the last strip of a row and are never summed.
     {
         // flat loops over (row, sample) with 8 loads in flight per lane
@@ -129,7 +136,7 @@ __global__ void __launch_bounds__(256) shake_ssd_kernel(ShakeK k) {
         const uint32_t nt = blockDim.x, tid = threadIdx.x;
         const uint64_t rstride = (uint64_t)k.stack_w * C;
         const uint32_t rlen = (uint32_t)((2 * r + 2 * s + 1) * C), rtotal = (uint32_t)(srows * pitch);
-        const uint64_t fbase = k.pos_off[f] + ((uint64_t)(cy - s - r + c0) * k.stack_w + (cx - s - r)) * C;
+        const uint64_t fbase = k.pos_off[f] + ((uint64_t)(cy - s - r + oy0 + c0) * k.stack_w + (cx - s - r)) * C;
         for (uint32_t i0 = 0; i0 < rtotal; i0 += U * nt) {
             uint32_t v[U];
 #pragma unroll
@@ -163,8 +170,9 @@ __global__ void __launch_bounds__(256) shake_ssd_kernel(ShakeK k) {
                 }
             }
         }
-        // sum of squared window samples, once per (anchor, chunk): frame-independent
-        if (f == 1 && wsq != 0.0) atomicAdd(k.sums + (uint64_t)k.n * ssize * ssize, (unsigned long long)wsq);
+        // sum of squared window samples, once per (anchor, window chunk): frame-independent
+        if (f == 1 && oc == 0 && wsq != 0.0)
+            atomicAdd(k.sums + (uint64_t)k.n * ssize * ssize, (unsigned long long)wsq);
     }
     __syncthreads();
     // rq[y][x] = sum over the window width of squared region samples starting at pixel x:
@@ -192,14 +200,14 @@ __global__ void __launch_bounds__(256) shake_ssd_kernel(ShakeK k) {
         }
     }
     __syncthreads();
-    const int32_t nstrips = k.spr * ssize;
+    const int32_t nstrips = k.spr * orows;
     const int32_t ntasks = k.strips_pad * k.nsub;
     const uint64_t noff = (uint64_t)ssize * ssize;
     for (int32_t t = threadIdx.x; t < ntasks; t += blockDim.x) {
         // strips_pad is a multiple of 64, so a wave shares its sub-chunk (and window reads)
         const int32_t sub = t / k.strips_pad, strip = t - sub * k.strips_pad;
         if (strip >= nstrips) continue;
-        const int32_t oy = strip / k.spr, ox0 = (strip - oy * k.spr) * K;
+        const int32_t oy = strip / k.spr, ox0 = (strip - oy * k.spr) * K;  // oy within the chunk
         const int32_t lo = sub * crows / k.nsub, hi = (sub + 1) * crows / k.nsub;
         double acc[K], sq[K];
 #pragma unroll
@@ -219,7 +227,7 @@ __global__ void __launch_bounds__(256) shake_ssd_kernel(ShakeK k) {
             for (int j = 0; j < K; ++j)
                 if (ox0 + j < ssize) sq[j] += rq[(oy + y) * ssize + ox0 + j];
         }
-        unsigned long long* out = k.sums + (uint64_t)f * noff + (uint64_t)oy * ssize + ox0;
+        unsigned long long* out = k.sums + (uint64_t)f * noff + (uint64_t)(oy0 + oy) * ssize + ox0;
 #pragma unroll
         for (int j = 0; j < K; ++j)
             if (ox0 + j < ssize)

@@ -65,40 +65,53 @@ extern "C" CP_API int cp_shake(cp_ctx* ctx, const cp_stack* st, const uint32_t* 
     k.offsets = (int32_t*)offs.ptr;
     hipError_t e = hipMemsetAsync(sums.ptr, 0, sums_bytes, hs);
     // Launch plan: strips of kShakeStrip offsets per lane, whole waves per row sub-chunk,
-    // about 8 window rows per sub-chunk, as many rows per workgroup as LDS holds.
+    // about 8 window rows per sub-chunk and all candidate rows per workgroup; where that
+    // does not fit LDS, the larger of the two row counts shrinks.
     constexpr int32_t K = kShakeStrip;
     const int32_t size = (int32_t)(2 * r + 1), C = (int32_t)st->channels;
     k.spr = (ssize + K - 1) / K;
-    k.strips_pad = (k.spr * ssize + 63) / 64 * 64;
-    int32_t block = 256;
-    if (k.strips_pad <= 256) {
-        k.nsub = 256 / k.strips_pad;
-        block = k.strips_pad * k.nsub;
-    } else {
-        k.nsub = 1;
-    }
     k.pitch_px = k.spr * K + (int32_t)(2 * r);
+    int32_t block = 256;
+    auto set_lanes = [&](int32_t orows) {
+        k.strips_pad = (k.spr * orows + 63) / 64 * 64;
+        k.nsub = k.strips_pad <= 256 ? 256 / k.strips_pad : 1;
+        block = k.strips_pad <= 256 ? k.strips_pad * k.nsub : 256;
+    };
     // Exact f64 sums: each lane adds ceil(rows / nsub) * size * C products below maxsq.
     const double maxsq = st->bps == 2 ? 65535.0 * 65535.0 : 255.0 * 255.0;
-    auto fits = [&](int32_t rows) {
+    auto fits = [&](int32_t rows, int32_t orows) {
+        set_lanes(orows);
         const double terms = (double)((rows + k.nsub - 1) / k.nsub) * size * C;
-        return shake_lds(rows, (int32_t)r, (int32_t)s, k.pitch_px, C).bytes <= 64 * 1024 &&
+        return shake_lds(rows, orows, (int32_t)r, (int32_t)s, k.pitch_px, C).bytes <= 64 * 1024 &&
                terms * maxsq < 9007199254740992.0;
     };
+    int32_t orows = ssize;
+    set_lanes(orows);
     const int32_t want = std::min<int32_t>(size, 8 * k.nsub);
     const int32_t even_chunks = (size + want - 1) / want;
     int32_t rows = (size + even_chunks - 1) / even_chunks;
-    while (rows > 1 && !fits(rows)) --rows;
+    while (!fits(rows, orows) && (rows > 1 || orows > 1)) {
+        if (orows > rows) --orows;
+        else --rows;
+    }
+    const bool ok = fits(rows, orows);  // also leaves the lane split of the final orows in k
     k.chunk_rows = rows;
+    k.cand_rows = orows;
+    k.cand_chunks = (ssize + orows - 1) / orows;
     const uint32_t nchunk = (uint32_t)((size + rows - 1) / rows);
-    const size_t lds = shake_lds(rows, (int32_t)r, (int32_t)s, k.pitch_px, C).bytes;
-    if (e == hipSuccess && (!fits(rows) || n_anchors > 65535)) {
+    const size_t lds = shake_lds(rows, orows, (int32_t)r, (int32_t)s, k.pitch_px, C).bytes;
+    if (e == hipSuccess && !ok) {
         e = hipErrorInvalidValue;
-        rc = fail(CP_ERR_INVALID_ARG, "anchor search region needs %zu bytes of LDS per window row (64 KiB at most)",
-                  (size_t)shake_lds(1, (int32_t)r, (int32_t)s, k.pitch_px, C).bytes);
+        rc = fail(CP_ERR_INVALID_ARG,
+                  "anchor search needs %zu bytes of LDS for one window row and one candidate row (64 KiB at most)",
+                  (size_t)shake_lds(1, 1, (int32_t)r, (int32_t)s, k.pitch_px, C).bytes);
+    } else if (e == hipSuccess && (uint64_t)n_anchors * k.cand_chunks > 65535) {
+        e = hipErrorInvalidValue;
+        rc = fail(CP_ERR_INVALID_ARG, "%u anchors x %d candidate-row chunks exceed 65535 workgroup rows",
+                  n_anchors, k.cand_chunks);
     }
     if (e == hipSuccess) {
-        dim3 grid(n - 1, n_anchors, nchunk), blk(block);
+        dim3 grid(n - 1, n_anchors * (uint32_t)k.cand_chunks, nchunk), blk(block);
         auto launch = [&](auto kern) { hipLaunchKernelGGL(kern, grid, blk, lds, hs, k); };
         const bool wide = st->bps == 2;
         switch (st->channels) {

@@ -186,7 +186,12 @@ CP_API int cp_simple_host(cp_ctx* ctx, const void* const* host_frames, uint32_t 
                           uint32_t bytes_per_sample, const cp_frames* sel, const float weights[4],
                           const cp_fade* fade, int32_t darker, void* out_px, uint64_t band_bytes);
 /* ShakeAnalyzer::analyze (src/shake.rs:190-313) over frame positions; offsets receives
- * (dx, dy) per position, position 0 = (0, 0).  frames may be NULL (slots 0..n-1). */
+ * (dx, dy) per position, position 0 = (0, 0).  frames may be NULL (slots 0..n-1).
+ * Anchors whose window or search region leaves the image fail with
+ * CP_ERR_ANCHOR_OUT_OF_RANGE (the reference panics); CP_ERR_INVALID_ARG when one window
+ * row against one candidate row (about 12r + 12s bytes of region row, 12r of window row and
+ * 8(2s+1) of box sums for 3 channels) exceeds 64 KiB of LDS: r up to about 1,800 or s up to
+ * about 2,300 for RGB. */
 CP_API int cp_shake(cp_ctx* ctx, const cp_stack* st, const uint32_t* frames, uint32_t n,
                     const int32_t* anchors_xy, uint32_t n_anchors, uint32_t anchor_radius,
                     uint32_t search_radius, int32_t* offsets);

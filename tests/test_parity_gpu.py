@@ -304,11 +304,12 @@ def test_shake_u8_wrapping_sum_matches(ctx):
 
 
 @pytest.mark.parametrize("c,bits,r,s", [(1, 8, 4, 1), (2, 16, 3, 2), (4, 8, 6, 3), (4, 16, 9, 7),
-                                         (3, 16, 12, 5), (1, 16, 20, 0), (3, 8, 0, 4)])
+                                         (3, 16, 12, 5), (1, 16, 20, 0), (3, 8, 0, 4), (3, 16, 10, 32),
+                                         (3, 8, 3, 40), (4, 16, 60, 12)])
 def test_shake_unstructured_frames(ctx, c, bits, r, s):
     """Random frames: every candidate's sum differs, so the arg-min exposes any wrong term.
-    Covers partial offset strips (2s+1 not a multiple of 5), several row chunks and 1-4
-    channels."""
+    Covers partial offset strips (2s+1 not a multiple of 5), several window-row chunks,
+    candidate rows split over workgroups (large s, or large r with s) and 1-4 channels."""
     rng = np.random.default_rng(100 + c * 7 + r + s)
     hi = (1 << bits) - 1
     h, w = 2 * (r + s) + 17, 2 * (r + s) + 23
