@@ -12,7 +12,8 @@
 //
 // Work split.  A workgroup owns one (frame, anchor, chunk of candidate rows, chunk of window
 // rows): it stages the frame-f search region those rows reach (window chunk + candidate
-// chunk - 1 rows, u16) and the frame-0 window rows (f32) in LDS.  Each candidate's sum is expanded as
+// chunk - 1 rows, u16) and the frame-0 window rows (f32) in LDS.  Each candidate's sum is
+// expanded as
 //   sum (w - v)^2 = sum w^2 + sum v^2 - 2 sum w*v,
 // all three exact integers.  sum w^2 is one constant for every frame and candidate: the
 // frame-1 workgroups add it to a spare slot of the table, and the arg-min kernel adds it back.
