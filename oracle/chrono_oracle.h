@@ -17,18 +17,25 @@
  *   or_blend_into_u8              commented reference KAT src/color.rs:51-64
  *   or_outlier                    brute-force invariants (mask <=> dist^2 >= min^2,
  *                                 first/last/extreme selection, darker/lighter laws),
- *                                 numpy float32 re-evaluation of the blend formulas.
- *                                 The background-average, forward/backward and
- *                                 average-outlier formulas have no independent pin
- *                                 beyond that numpy re-evaluation ("parity unpinned"
- *                                 against the reference binary, which cannot be built).
+ *                                 numpy float32 re-evaluation of the distances; the
+ *                                 background average with and without outliers against
+ *                                 the exact rational of the reference's expression
+ *                                 (blends switched off by a zero fade); the average,
+ *                                 forward and backward modes at full blend against the
+ *                                 exact rounded outlier mean and the last / first
+ *                                 outlier.  Blend values strictly between 0 and 1 in the
+ *                                 forward/backward and average modes: parity unpinned
+ *                                 beyond the blend KAT (the reference binary cannot be
+ *                                 built here).
  *   or_simple                     numpy argmin/argmax (first index on ties) without fade
  *   or_shake                      brute-force int64 SSD, recovery of seeded translations
  *   or_crop_create                closed form src/shake.rs:136-176
  *   or_video_windows              hand-derived window tables src/main.rs:230-286
- *   random background             parity unpinned: the reference draws from an
+ *   random background             values parity unpinned: the reference draws from an
  *                                 OS-seeded ThreadRng; this restatement uses a
- *                                 seeded counter hash (same candidate logic).
+ *                                 seeded counter hash.  Pinned: the candidate logic
+ *                                 (support of the draws with and without outliers, the
+ *                                 swap quirk) and uniform frequencies.
  */
 #ifndef CHRONO_ORACLE_H
 #define CHRONO_ORACLE_H

@@ -186,6 +186,117 @@ def test_background_average_without_outliers_is_rounded_mean():
     assert np.all(mask[..., :3] == 0)
 
 
+def _round_half_away(num, den):
+    """Exact half-away-from-zero rounding of num/den for non-negative integers."""
+    return (2 * num + den) // (2 * den)
+
+
+def _outlier_sets(x, thr_min):
+    med = np.median(x.astype(np.float64), axis=0).astype(np.float32)
+    d = _numpy_distances(x, med, np.zeros_like(med), (1, 1, 1, 1), True)
+    return d >= F32(thr_min) * F32(thr_min)
+
+
+ZERO_FADE = dict(repeat=False, absolute=True, frames=[0, 1], values=[0.0, 0.0])
+
+
+def test_background_average_with_outliers_closed_form():
+    """A fade that is 0 on every frame makes every blend value 0 (src/color.rs:5: blend <= 0
+    leaves the pixel), so the output is the background itself.  With k outliers of N samples,
+    sum S and outlier sum O, src/chrono.rs:306-331 evaluates mean * N/(N-k) - O/N, i.e.
+    S/(N-k) - O/N (not the mean of the other samples; the source marks the equation "TODO:
+    check"), rounded half away and saturated to u8.  The exact rational of that expression
+    has denominator N (N-k), so it sits at least 1/(2N(N-k)) from a half unless it is one;
+    for u8 and N = 25 the f32 evaluation errs by < 255 N 2^-22, well inside that gap, so only
+    exact halves (skipped) could round either way."""
+    rng = np.random.default_rng(21)
+    thr = O.threshold(True, 0.05, 0.2)
+    checked, ks = 0, []
+    for frac in (0.3, 0.04):  # many-outlier pixels, and single-outlier ones (their own branch)
+        x = _stack(rng, 25, 6, 7, 3, 8, outlier_frac=frac)
+        img, mask, _, st = O.outlier(x, thr, 2, 2, fade=O.Fade(**ZERO_FADE), stats=True)
+        out = _outlier_sets(x, thr.min)
+        assert np.array_equal(st["num_outliers"], out.sum(axis=0))
+        assert np.all(mask == 0)
+        n = x.shape[0]
+        for yy in range(x.shape[1]):
+            for xx in range(x.shape[2]):
+                k = int(out[:, yy, xx].sum())
+                if k == n:
+                    continue
+                ks.append(k)
+                for ch in range(3):
+                    tot = int(x[:, yy, xx, ch].astype(np.int64).sum())
+                    osum = int(x[out[:, yy, xx], yy, xx, ch].astype(np.int64).sum())
+                    num, den = tot * n - osum * (n - k), n * (n - k)
+                    if (2 * num) % den == 0 and (2 * num // den) % 2 != 0:
+                        continue  # exact half
+                    want = 0 if num < 0 else min(255, _round_half_away(num, den))
+                    assert img[yy, xx, ch] == want, (frac, yy, xx, ch, k)
+                    checked += 1
+    assert checked > 200 and ks.count(0) > 0 and ks.count(1) > 3 and sum(k > 1 for k in ks) > 5
+
+
+def test_average_outlier_mode_with_full_blend_is_rounded_outlier_mean():
+    """src/chrono.rs:430-466: with two or more outliers the blended sample is the rounded f32
+    mean of the outliers' samples; min == max makes every blend value 1, which copies it
+    (src/color.rs:6-9).  4-channel stacks: the reference blends its 4-entry mean into the
+    pixel, which only fits RGBA at blend 1.  For u8 the f32 quotient of exact sums rounds like
+    the exact rational (halves included: they are exact in f32)."""
+    rng = np.random.default_rng(23)
+    x = _stack(rng, 30, 5, 6, 4, 8, outlier_frac=0.35)
+    thr = O.threshold(True, 0.05, 0.05)
+    img, mask, _, st = O.outlier(x, thr, 0, 3, stats=True)
+    out = _outlier_sets(x, thr.min)
+    assert np.array_equal(st["num_outliers"], out.sum(axis=0))
+    multi = 0
+    for yy in range(x.shape[1]):
+        for xx in range(x.shape[2]):
+            ts = np.nonzero(out[:, yy, xx])[0]
+            if len(ts) == 0:
+                assert np.array_equal(img[yy, xx], x[0, yy, xx]) and np.all(mask[yy, xx] == 0)
+                continue
+            assert np.all(mask[yy, xx] == 255)
+            if len(ts) == 1:
+                assert np.array_equal(img[yy, xx], x[ts[0], yy, xx])
+                continue
+            multi += 1
+            for ch in range(4):
+                osum = int(x[ts, yy, xx, ch].astype(np.int64).sum())
+                assert img[yy, xx, ch] == _round_half_away(osum, len(ts)), (yy, xx, ch)
+    assert multi > 5
+
+
+def test_random_background_draws_uniformly_from_the_candidates():
+    """src/chrono.rs:344-349, 530-555 with every blend value 0 (the output is the background
+    frame).  The reference draws from an OS-seeded ThreadRng, so single values cannot be
+    pinned; what is fixed: no outliers -> gen_range(0, N) over every frame; outliers {1, 4}
+    of N = 6 -> the swap loop leaves positions [0, 4) holding frames {0, 5, 2, 3} (swap(1, 5),
+    then swap(4, 4)), and the draw is uniform over them.  Many pixels with one pattern check
+    the support exactly and the frequencies to within 25%."""
+    n, h, w = 6, 40, 40
+    x = np.full((n, h, w, 1), 100, np.uint8)
+    for t in range(n):
+        x[t, ..., 0] += t  # distinct per frame, all within the threshold
+    img, mask, _, st = O.outlier(x, O.threshold(True, 0.05, 0.2), 1, 2, fade=O.Fade(**ZERO_FADE),
+                                 stats=True, seed=7)
+    assert np.all(st["num_outliers"] == 0)
+    bg = st["bg_index"].reshape(-1)
+    assert np.array_equal(img[..., 0].reshape(-1), 100 + bg)
+    counts = np.bincount(bg, minlength=n)
+    assert len(counts) == n and np.all(np.abs(counts - h * w / n) < 0.25 * h * w / n)
+    x[1, ..., 0] = 250
+    x[4, ..., 0] = 0
+    img, mask, _, st = O.outlier(x, O.threshold(True, 0.05, 0.2), 1, 2, fade=O.Fade(**ZERO_FADE),
+                                 stats=True, seed=7)
+    assert np.all(st["num_outliers"] == 2)
+    bg = st["bg_index"].reshape(-1)
+    assert np.array_equal(img[..., 0].reshape(-1), x[bg, 0, 0, 0])
+    counts = np.bincount(bg, minlength=n)
+    assert set(np.nonzero(counts)[0].tolist()) == {0, 2, 3, 5}
+    assert np.all(np.abs(counts[[0, 2, 3, 5]] - h * w / 4) < 0.25 * h * w / 4)
+
+
 def test_background_first_and_warning_for_all_outlier_pixel():
     x = np.zeros((4, 1, 1, 3), np.uint8)
     x[:, 0, 0, 0] = [0, 0, 255, 255]  # median 127.5: every frame is 127.5 away
