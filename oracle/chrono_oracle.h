@@ -23,10 +23,12 @@
  *                                 (blends switched off by a zero fade); the average,
  *                                 forward and backward modes at full blend against the
  *                                 exact rounded outlier mean and the last / first
- *                                 outlier.  Blend values strictly between 0 and 1 in the
- *                                 forward/backward and average modes: parity unpinned
- *                                 beyond the blend KAT (the reference binary cannot be
- *                                 built here).
+ *                                 outlier; partial blends with per-frame fades of
+ *                                 1/4, 1/2, 3/4 (exact in f32) against exact rational
+ *                                 convex steps in frame / reverse order, and the
+ *                                 average mode's frame-0 fade.  The f32 rounding of
+ *                                 other partial blends follows the reference's
+ *                                 a + (b - a) t and is checked by the blend KAT only.
  *   or_simple                     numpy argmin/argmax (first index on ties) without fade
  *   or_shake                      brute-force int64 SSD, recovery of seeded translations
  *   or_crop_create                closed form src/shake.rs:136-176

@@ -267,6 +267,82 @@ def test_average_outlier_mode_with_full_blend_is_rounded_outlier_mean():
     assert multi > 5
 
 
+def test_partial_blends_with_dyadic_fades_are_exact_convex_steps():
+    """src/chrono.rs:379-428 with src/color.rs:4-44.  Every outlier blends fully (min = max)
+    and the fade gives each frame a blend of 1/4, 1/2 or 3/4, so each step a + (b - a) t of
+    u8 values is exact in f32 while the value fits 24 bits (up to 8 outliers): forward and
+    backward must equal the exact rational sequence of convex steps in frame order / reverse
+    order, rounded half away, and the mask round((1 - prod(1 - t)) * 255); one outlier must
+    give round(bg + (s - bg) t) and round(t * 255).  Background first."""
+    from fractions import Fraction as Fr
+    rng = np.random.default_rng(29)
+    n = 20
+    x = _stack(rng, n, 6, 7, 3, 8, outlier_frac=0.2)
+    tvals = rng.choice([0.25, 0.5, 0.75], n)
+    fade = O.Fade(False, True, list(range(n)), [float(t) for t in tvals])
+    thr = O.threshold(True, 0.05, 0.05)
+    out = _outlier_sets(x, thr.min)
+    seen = {1: 0, 2: 0}
+    for mode in (4, 5):
+        img, mask, _, st = O.outlier(x, thr, 0, mode, fade=fade, stats=True)
+        assert np.array_equal(st["num_outliers"], out.sum(axis=0))
+        for yy in range(x.shape[1]):
+            for xx in range(x.shape[2]):
+                ts = list(np.nonzero(out[:, yy, xx])[0])
+                k = len(ts)
+                if k == 0 or k == n or k > 8:
+                    continue
+                bg = x[[t for t in range(n) if t not in ts][0], yy, xx]
+                order = ts if mode == 4 else ts[::-1]
+                inv = Fr(1)
+                for ch in range(3):
+                    a = Fr(int(bg[ch]))
+                    for t in order:
+                        a = a + (int(x[t, yy, xx, ch]) - a) * Fr(float(tvals[t]))
+                    want = _round_half_away(a.numerator, a.denominator)
+                    assert img[yy, xx, ch] == want, (mode, yy, xx, ch, k)
+                for t in order:
+                    inv *= 1 - Fr(float(tvals[t]))
+                m = (1 - inv) * 255
+                assert np.all(mask[yy, xx] == _round_half_away(m.numerator, m.denominator)), (mode, yy, xx)
+                seen[min(k, 2)] += 1
+    assert seen[1] > 3 and seen[2] > 10
+
+
+def test_average_mode_partial_blend_uses_frame_zero_fade():
+    """src/chrono.rs:429-481: with two or more outliers the average mode blends the rounded
+    outlier mean with the fade of frame 0 (the tuple's index is 0), with one outlier that
+    outlier with its own frame's fade; full blend values (min = max) and dyadic fades make
+    round(bg + (s - bg) t) and round(t * 255) exact.  A partial blend zips only C entries,
+    so RGB is fine here."""
+    rng = np.random.default_rng(31)
+    n = 20
+    tvals = rng.choice([0.25, 0.5, 0.75], n)
+    fade = O.Fade(False, True, list(range(n)), [float(t) for t in tvals])
+    thr = O.threshold(True, 0.05, 0.05)
+    seen = {1: 0, 2: 0}
+    for frac in (0.25, 0.04):
+        x = _stack(rng, n, 6, 7, 3, 8, outlier_frac=frac)
+        out = _outlier_sets(x, thr.min)
+        img, mask, _, st = O.outlier(x, thr, 0, 3, fade=fade, stats=True)
+        for yy in range(x.shape[1]):
+            for xx in range(x.shape[2]):
+                ts = list(np.nonzero(out[:, yy, xx])[0])
+                k = len(ts)
+                if k == 0 or k == n:
+                    continue
+                bg = x[[t for t in range(n) if t not in ts][0], yy, xx].astype(np.int64)
+                t = tvals[ts[0]] if k == 1 else tvals[0]
+                for ch in range(3):
+                    sm = int(x[ts[0], yy, xx, ch]) if k == 1 else _round_half_away(
+                        int(x[ts, yy, xx, ch].astype(np.int64).sum()), k)
+                    v = bg[ch] * 4 + (sm - bg[ch]) * int(t * 4)  # 4 * (bg + (s - bg) t)
+                    assert img[yy, xx, ch] == _round_half_away(v, 4), (frac, yy, xx, ch, k)
+                assert np.all(mask[yy, xx] == _round_half_away(int(t * 4) * 255, 4))
+                seen[min(k, 2)] += 1
+    assert seen[1] > 3 and seen[2] > 10
+
+
 def test_random_background_draws_uniformly_from_the_candidates():
     """src/chrono.rs:344-349, 530-555 with every blend value 0 (the output is the background
     frame).  The reference draws from an OS-seeded ThreadRng, so single values cannot be

@@ -217,6 +217,25 @@ def test_outlier_frame_counts(ctx, n):
         assert_same(gpu, ref, f"n={n} rel")
 
 
+@pytest.mark.parametrize("bits", [8, 16])
+def test_outlier_pinned_blend_scenarios(ctx, bits):
+    """The oracle's blend pins run on the GPU too: a zero fade (output = background) under
+    every background mode, and per-frame fades of 1/4, 1/2, 3/4 with full blend values in the
+    average, forward and backward modes."""
+    n = 20
+    zero = (False, True, [(0, 0.0), (1, 0.0)])
+    rng = np.random.default_rng(37)
+    dyadic = (False, True, [(t, float(v)) for t, v in enumerate(rng.choice([0.25, 0.5, 0.75], n))])
+    for frac in (0.25, 0.04):
+        x = make_stack(int(frac * 100) + bits, n, 9, 13, 3, bits, outlier_frac=frac)
+        for bg in (0, 1, 2, 3):
+            gpu, ref = run_both(ctx, x, True, 0.05, 0.2, bg, 2, fade=zero)
+            assert_same(gpu, ref, f"zero fade bg {bg} frac {frac}")
+        for sel in (3, 4, 5):
+            gpu, ref = run_both(ctx, x, True, 0.05, 0.05, 0, sel, fade=dyadic)
+            assert_same(gpu, ref, f"dyadic fade sel {sel} frac {frac}")
+
+
 def test_outlier_fade_windows_crop_sample(ctx):
     x = make_stack(900, 30, 12, 16, 3)
     fade = (False, True, [(2, 0.2), (10, 1.0), (20, 0.4)])
