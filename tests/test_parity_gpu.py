@@ -286,6 +286,29 @@ def test_simple_parity(ctx, darker, bits, c):
         assert np.array_equal(img, ref)
 
 
+@pytest.mark.parametrize("bits,c", [(8, 1), (8, 2), (8, 3), (8, 4), (16, 1), (16, 2), (16, 3), (16, 4)])
+def test_simple_widths_and_crops(ctx, bits, c):
+    """Every row alignment the lanes' dword reads can meet: widths 1..9 and 61, crop origins
+    that move each frame's row start by one sample, and a partial last lane."""
+    rng = np.random.default_rng(60 + bits + c)
+    n, h = 11, 5
+    for w in list(range(1, 10)) + [61]:
+        x = make_stack(int(rng.integers(1 << 30)), n, h, w + 3, c, bits=bits, outlier_frac=0.3)
+        st = cpa.Stack.from_array(ctx, x)
+        offs = [(int(rng.integers(0, 4)), int(rng.integers(0, 2))) for _ in range(n)]
+        offs[0] = (0, 0)
+        crop = cpa.Crop.create([(dx - 3, dy - 1) for dx, dy in offs], w + 3, h)
+        ocrop = O.crop_create([(dx - 3, dy - 1) for dx, dy in offs], w + 3, h)
+        for darker in (True, False):
+            img = cpa.SimpleProcessor((1, 0.5, 2, 1)[:4], None, darker).process(st)
+            assert np.array_equal(img, O.simple(x, (1, 0.5, 2, 1), None, darker)), (w, darker)
+            if crop is not None:
+                img = cpa.SimpleProcessor(darker=darker).process(st, crop=crop)
+                ref = O.simple(x, darker=darker, crop_xy=ocrop[0], out_w=ocrop[1], out_h=ocrop[2])
+                assert np.array_equal(img, ref), ("crop", w, darker, offs)
+        st.close()
+
+
 # --------------------------------------------------------------------------- shake
 
 def textured(n, h, w, shifts, c=3, bits=8, seed=0):
