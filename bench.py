@@ -68,6 +68,12 @@ WORKLOADS = {
                    base_lo=40, base_hi=200, sigma=3.0, n_disks=3, disk_r_min=10, disk_r_max=40,
                    disk_val_lo=15, disk_val_hi=25, blob_permille=100, blob_r=30, blob_val=230),
         shake=None, mode="lighter"),
+    "darker_4k_rgb16_256f": dict(
+        scene=dict(seed=2, width=3840, height=2160, channels=3, bytes_per_sample=2, n_frames=256,
+                   base_lo=8000, base_hi=50000, sigma=200.0, n_disks=3, disk_r_min=20, disk_r_max=40,
+                   disk_val_lo=2000, disk_val_hi=4000, shake_max=8, texture_amp=3000,
+                   disk_y_lo=540, disk_y_hi=1620),
+        shake=None, mode="darker"),
     "outlier_640x480_rgb8_16f": dict(
         scene=dict(seed=0, width=640, height=480, channels=3, bytes_per_sample=1, n_frames=16,
                    base_lo=40, base_hi=200, sigma=3.0, n_disks=1, disk_r_min=20, disk_r_max=20,
