@@ -383,6 +383,23 @@ def test_shake_ties_keep_first_offset(ctx, bits):
     assert off[1:].tolist() == [[-7, -7]] * 3
 
 
+def test_shake_frame_selection_single_frame_and_no_anchors(ctx):
+    """Frame slots in any order (the reference's image list after --frames), one frame only
+    (offset (0, 0)), and no anchors (every sum 0: min_by_key keeps offset (-s, -s))."""
+    shifts = [(0, 0), (2, -1), (-3, 3), (1, 0), (0, -2), (4, 4)]
+    x = textured(len(shifts), 48, 64, shifts, bits=16, seed=4)
+    st = cpa.Stack.from_array(ctx, x)
+    anchors = [(20, 16), (40, 30)]
+    slots = [3, 0, 5, 1]
+    off = cpa.ShakeAnalyzer().analyze(st, anchors, 5, 4, frames=slots)
+    assert np.array_equal(off, O.shake(x, anchors, 5, 4, slots=slots))
+    off = cpa.ShakeAnalyzer().analyze(st, anchors, 5, 4, frames=[2])
+    assert off.tolist() == [[0, 0]] and np.array_equal(off, O.shake(x, anchors, 5, 4, slots=[2]))
+    off = cpa.ShakeAnalyzer().analyze(st, [], 5, 4)
+    assert np.array_equal(off, O.shake(x, [], 5, 4))
+    assert off[1:].tolist() == [[-4, -4]] * (len(shifts) - 1)
+
+
 def test_shake_anchor_errors(ctx):
     x = textured(2, 30, 30, [(0, 0), (1, 1)])
     st = cpa.Stack.from_array(ctx, x)
