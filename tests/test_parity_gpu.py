@@ -90,8 +90,11 @@ def test_outlier_u16_wrapped_stack(ctx, c, w, shift):
 @pytest.mark.parametrize("n", [16, 32, 64, 128, 256])
 @pytest.mark.parametrize("c", [3, 4])
 def test_outlier_u16_full_tiles(ctx, n, c):
-    """Frame counts that fill the tile exactly take the 7-bit window search around the mean
-    estimate; noisy pixels, pixels with runs of outliers and constant pixels all stay exact."""
+    """Frame counts that fill the tile exactly: absolute thresholds take the byte search in a
+    256-wide window around the remedian estimate (exact two-level search where the result
+    lands on the window's edge), relative ones the exact two-level search for the median and
+    both quartiles; noisy pixels, pixels with runs of outliers, constant pixels and pixels at
+    either end of the range all stay exact."""
     for frac, seed in ((0.02, 1), (0.3, 2)):
         x = make_stack(1300 + 10 * n + c + seed, n, 5, 70, c, bits=16, outlier_frac=frac)
         x[:, 1, 2:6, :] = 40000                        # constant pixels
@@ -101,6 +104,23 @@ def test_outlier_u16_full_tiles(ctx, n, c):
         for sel in (2, 4):
             gpu, ref = run_both(ctx, x, True, 0.05, 0.2, 0, sel)
             assert_same(gpu, ref, f"n={n} c={c} frac={frac} sel={sel}")
+        gpu, ref = run_both(ctx, x, False, 2.0, 4.0, 2, 3)
+        assert_same(gpu, ref, f"n={n} c={c} frac={frac} rel")
+
+
+@pytest.mark.parametrize("n", [16, 32, 64, 128, 256, 512])
+@pytest.mark.parametrize("c", [3, 4])
+def test_outlier_u8_full_tiles(ctx, n, c):
+    """8-bit stacks whose frame count fills a tile shape exactly, with constant pixels, a run
+    of outliers and pixels at either end of the range, for absolute and relative thresholds."""
+    x = make_stack(1700 + n + c, n, 5, 40, c, bits=8, outlier_frac=0.1)
+    x[:, 1, 2:6, :] = 200
+    x[: n // 3, 2, 2:6, :] = 10
+    x[:, 3, 2:6, :] = 255 - (np.arange(n) % 3)[:, None, None]
+    x[:, 4, 2:6, :] = np.arange(n)[:, None, None] % 2
+    for absolute, lo, hi, bg, sel in ((True, 0.05, 0.2, 0, 4), (False, 2.0, 4.0, 2, 3), (False, 1.0, 1.5, 3, 5)):
+        gpu, ref = run_both(ctx, x, absolute, lo, hi, bg, sel)
+        assert_same(gpu, ref, f"n={n} c={c} abs={absolute} sel={sel}")
 
 
 @pytest.mark.parametrize("bits,c", [(8, 3), (16, 3), (16, 4), (8, 1)])
