@@ -237,6 +237,29 @@ def test_outlier_frame_counts(ctx, n):
         assert_same(gpu, ref, f"n={n} rel")
 
 
+@pytest.mark.parametrize("bits,n", [(8, 513), (8, 700), (16, 300), (16, 520)])
+def test_outlier_tall_stacks_take_the_generic_kernel(ctx, bits, n):
+    """Stacks taller than the register tiles hold (over 512 frames), and 16-bit average modes
+    whose integer sums could leave f32's exact range (N * 65535 >= 2^24), run on the generic
+    kernel; every mode family stays exact there."""
+    x = make_stack(2100 + n + bits, n, 4, 9, 3, bits=bits, outlier_frac=0.1)
+    for absolute, lo, hi, bg, sel in ((True, 0.05, 0.2, 0, 4), (False, 2.0, 4.0, 2, 3),
+                                      (True, 0.05, 0.2, 1, 2), (False, 1.0, 1.5, 3, 5)):
+        gpu, ref = run_both(ctx, x, absolute, lo, hi, bg, sel)
+        assert_same(gpu, ref, f"bits={bits} n={n} abs={absolute} bg={bg} sel={sel}")
+
+
+@pytest.mark.parametrize("bits,c", [(16, 3), (16, 1), (8, 2), (16, 4)])
+def test_outlier_generic_kernel_modes(ctx, bits, c):
+    """The generic kernel forced on 16-bit and 1/2/4-channel stacks across the mode families
+    (it is also what 1- and 2-channel stacks always use)."""
+    x = make_stack(2300 + bits + c, 23, 6, 11, c, bits=bits)
+    for bg, sel in ((0, 0), (1, 1), (2, 3), (3, 4), (0, 5), (2, 2)):
+        for absolute, lo, hi in ((True, 0.05, 0.2), (False, 2.0, 4.0)):
+            gpu, ref = run_both(ctx, x, absolute, lo, hi, bg, sel, force_generic=True)
+            assert_same(gpu, ref, f"bits={bits} c={c} bg={bg} sel={sel} abs={absolute}")
+
+
 @pytest.mark.parametrize("bits", [8, 16])
 def test_outlier_pinned_blend_scenarios(ctx, bits):
     """The oracle's blend pins run on the GPU too: a zero fade (output = background) under
