@@ -144,7 +144,7 @@ ABI_SYMBOLS = [
     "cp_stack_info_get", "cp_stack_destroy", "cp_threshold_new", "cp_threshold_blend_value",
     "cp_fade_table", "cp_fade_get", "cp_outlier", "cp_outlier_host", "cp_simple", "cp_simple_host", "cp_shake", "cp_crop_create",
     "cp_video_windows", "cp_batch_run", "cp_batch_create", "cp_batch_launch", "cp_batch_destroy",
-    "cp_autotune", "cp_server_create", "cp_server_submit", "cp_server_poll",
+    "cp_autotune", "cp_tile_shapes_set", "cp_server_create", "cp_server_submit", "cp_server_poll",
     "cp_server_wait", "cp_server_stats_get", "cp_server_destroy",
 ]
 
@@ -209,6 +209,7 @@ def load_library() -> C.CDLL:
         "cp_batch_launch": (C.c_int, [C.c_void_p]),
         "cp_batch_destroy": (C.c_int, [C.c_void_p]),
         "cp_autotune": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, P(C.c_int32)]),
+        "cp_tile_shapes_set": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, P(C.c_int32)]),
         "cp_server_create": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, P(C.c_void_p)]),
         "cp_server_submit": (C.c_int, [C.c_void_p, P(cp_request), P(C.c_uint64)]),
         "cp_server_stats_get": (C.c_int, [C.c_void_p, P(cp_server_stats)]),
@@ -890,6 +891,14 @@ class Batch:
 def batch_run(ctx: Context, jobs):
     arr = _job_array(list(jobs))
     _check(ctx.lib.cp_batch_run(ctx.handle, arr, len(arr), 0))
+
+
+def tile_shapes_set(ctx: Context, bytes_per_sample: int, channels: int, shapes):
+    """cp_tile_shapes_set: pins the tile shape per bucket; shapes are (lanes_per_pixel,
+    frames_per_lane, pixels_per_lane) tuples as autotune returns them, or None for the
+    built-in default of that bucket."""
+    arr = (C.c_int32 * 6)(*[0 if sh is None else (sh[2] << 16) | (sh[0] << 8) | sh[1] for sh in shapes])
+    _check(ctx.lib.cp_tile_shapes_set(ctx.handle, bytes_per_sample, channels, arr))
 
 
 def autotune(ctx: Context, bytes_per_sample: int, channels: int):

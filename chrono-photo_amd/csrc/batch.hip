@@ -285,6 +285,27 @@ __global__ void autotune_fill_kernel(uint8_t* data, uint32_t bps, uint64_t n_ele
     }
 }
 
+extern "C" CP_API int cp_tile_shapes_set(cp_ctx* ctx, uint32_t bps, uint32_t channels, const int32_t* shapes) {
+    if (!ctx || !shapes) return fail(CP_ERR_INVALID_ARG, "null argument");
+    if ((bps != 1 && bps != 2) || channels < 3 || channels > 4)
+        return fail(CP_ERR_INVALID_ARG, "tile shapes cover 8/16-bit stacks with 3 or 4 channels");
+    int8_t pick[kNumBuckets];
+    for (int b = 0; b < kNumBuckets; ++b) {
+        pick[b] = -1;
+        if (shapes[b] == 0) continue;
+        for (int c = 0; c < 2; ++c) {
+            const int sh = kBucketCand[bps - 1][b][c];
+            if (sh >= 0 && shapes[b] == ((kShapes[sh].X << 16) | (kShapes[sh].G << 8) | kShapes[sh].F))
+                pick[b] = (int8_t)sh;
+        }
+        if (pick[b] < 0)
+            return fail(CP_ERR_INVALID_ARG, "shape 0x%06x is not a candidate for frame bucket %d", shapes[b], b);
+    }
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    for (int b = 0; b < kNumBuckets; ++b) ctx->tuned[bps - 1][channels - 1][b] = pick[b];
+    return CP_OK;
+}
+
 extern "C" CP_API int cp_autotune(cp_ctx* ctx, uint32_t bps, uint32_t channels, int32_t* shapes) {
     if (!ctx) return fail(CP_ERR_INVALID_ARG, "null ctx");
     if ((bps != 1 && bps != 2) || channels < 3 || channels > 4)

@@ -538,8 +538,9 @@ __device__ __forceinline__ void finish_pixel(const OutlierK& k, uint32_t bx, uin
             num += __builtin_popcount(mm);
             if (first_idx == NONE && mm) first_idx = base + __builtin_ctz(mm);
             if (mm) last_idx = base + 31 - __builtin_clz(mm);
-            // valid non-outlier frames of this word
-            const int32_t nw = min((int32_t)N - (int32_t)base, 32);
+            // valid non-outlier frames of this word: a word holds min(32, F - 32w) frames
+            const int32_t span = F - 32 * w < 32 ? F - 32 * w : 32;
+            const int32_t nw = min((int32_t)N - (int32_t)base, span);
             const uint32_t vm = nw >= 32 ? 0xFFFFFFFFu : (nw <= 0 ? 0u : (1u << nw) - 1u);
             const uint32_t non = vm & ~mm;
             if (first_non == NONE && non) first_non = base + __builtin_ctz(non);

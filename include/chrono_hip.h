@@ -247,6 +247,12 @@ CP_API int cp_batch_run(cp_ctx* ctx, const cp_job* jobs, uint32_t n_jobs, uint32
  * (nullable, 6 entries) receives (pixels_per_lane << 16) | (lanes_per_pixel << 8) |
  * frames_per_lane per bucket. */
 CP_API int cp_autotune(cp_ctx* ctx, uint32_t bytes_per_sample, uint32_t channels, int32_t* shapes);
+/* Sets the context's tile shape per bucket without timing anything (no reference
+ * counterpart): shapes holds 6 entries in cp_autotune's encoding, for instance a result that
+ * cp_autotune returned in an earlier process, or 0 for the built-in default of that bucket.
+ * Each entry must be one of the bucket's two candidate shapes (CP_ERR_INVALID_ARG
+ * otherwise, and nothing is changed). */
+CP_API int cp_tile_shapes_set(cp_ctx* ctx, uint32_t bytes_per_sample, uint32_t channels, const int32_t* shapes);
 
 /* -------------------------------------------------------------------- serving
  * Continuous-batching request loop (no reference counterpart: the reference's video mode

@@ -237,6 +237,43 @@ def test_outlier_frame_counts(ctx, n):
         assert_same(gpu, ref, f"n={n} rel")
 
 
+# each frame bucket's two candidate tile shapes (lanes per pixel, frames per lane, pixels per lane)
+CANDIDATES = {
+    8: [[(1, 16, 1)], [(1, 32, 1), (2, 16, 1)], [(2, 32, 1), (4, 16, 1)], [(4, 32, 1), (8, 16, 1)],
+        [(8, 32, 1), (4, 64, 1)], [(8, 64, 1)]],
+    16: [[(1, 16, 2), (1, 16, 1)], [(1, 32, 2), (1, 32, 1)], [(2, 32, 2), (2, 32, 1)], [(4, 32, 2), (4, 32, 1)],
+         [(8, 32, 2), (8, 32, 1)], [(8, 64, 1)]],
+}
+
+
+@pytest.mark.parametrize("bits,c", [(8, 3), (8, 4), (16, 3), (16, 4)])
+def test_outlier_every_candidate_tile_shape(ctx, bits, c):
+    """Every tile shape the autotuner can pick, pinned with cp_tile_shapes_set: frame counts
+    that fill each bucket, a run of outliers longer than one 16-frame block (the background
+    is then the first frame after the run), constant and range-edge pixels."""
+    hi = (1 << bits) - 1
+    try:
+        for pick in (0, 1):
+            shapes = [cands[min(pick, len(cands) - 1)] for cands in CANDIDATES[bits]]
+            cpa.tile_shapes_set(ctx, bits // 8, c, shapes)
+            for n in (16, 32, 64, 128, 256) + ((512,) if bits == 8 else ()):
+                x = make_stack(2500 + n + c + bits, n, 4, 37, c, bits=bits, outlier_frac=0.05)
+                x[:, 1, 2:6, :] = hi // 2
+                x[: (n - 1) // 2, 2, 2:6, :] = hi // 40          # outliers over the first blocks
+                x[:, 3, 2:6, :] = hi - (np.arange(n) % 3)[:, None, None]
+                for absolute, lo, thr_hi, bg, sel in ((True, 0.05, 0.2, 0, 4), (False, 2.0, 4.0, 2, 3)):
+                    gpu, ref = run_both(ctx, x, absolute, lo, thr_hi, bg, sel)
+                    assert_same(gpu, ref, f"bits={bits} c={c} n={n} shape={shapes} abs={absolute}")
+    finally:
+        cpa.tile_shapes_set(ctx, bits // 8, c, [None] * 6)
+
+
+def test_tile_shapes_set_rejects_non_candidates(ctx):
+    with pytest.raises(cpa.ChronoError) as e:
+        cpa.tile_shapes_set(ctx, 1, 3, [None, (8, 64, 1), None, None, None, None])
+    assert e.value.code == cpa.CP_ERR_INVALID_ARG
+
+
 @pytest.mark.parametrize("bits,n", [(8, 513), (8, 700), (16, 300), (16, 520)])
 def test_outlier_tall_stacks_take_the_generic_kernel(ctx, bits, n):
     """Stacks taller than the register tiles hold (over 512 frames), and 16-bit average modes
