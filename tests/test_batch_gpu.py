@@ -103,6 +103,46 @@ def test_batch_against_oracle(ctx):
         assert np.array_equal(m, omask), spec
 
 
+CANDIDATES_U8 = [[(1, 16, 1)], [(1, 32, 1), (2, 16, 1)], [(2, 32, 1), (4, 16, 1)], [(4, 32, 1), (8, 16, 1)],
+                 [(8, 32, 1), (4, 64, 1)], [(8, 64, 1)]]
+
+
+def test_batch_with_every_candidate_shape_against_oracle(ctx):
+    """Batched groups under each pinned tile shape (cp_tile_shapes_set) against the oracle,
+    on stacks whose first frames are a run of outliers longer than a 16-frame block."""
+    rng = np.random.default_rng(9)
+    try:
+        for pick in (0, 1):
+            cpa.tile_shapes_set(ctx, 1, 3, [c[min(pick, len(c) - 1)] for c in CANDIDATES_U8])
+            jobs, keep = [], []
+            for i, n in enumerate((20, 40, 64, 100, 128, 200)):
+                x = rng.integers(60, 70, (n, 6, 33, 3))
+                x[: (n - 1) // 2, 2, 3:9, :] = 5
+                hit = rng.random((n, 6, 33)) < 0.05
+                x = np.where(hit[..., None], rng.integers(0, 256, x.shape), x).astype(np.uint8)
+                st = cpa.Stack.from_array(ctx, x)
+                px, mask = H.DeviceBuffer(x[0].size), H.DeviceBuffer(x[0].size)
+                absolute = i % 2 == 0
+                thr = cpa.Threshold.abs(0.05, 0.2) if absolute else cpa.Threshold.rel(2.0, 4.0)
+                sel = SELS[(i + pick) % 6]
+                proc = cpa.OutlierProcessor(thr, cpa.BackgroundMode.FIRST, sel)
+                jobs.append(cpa.Job(cpa.CP_MODE_OUTLIER, st, px.ptr, mask.ptr, processor=proc))
+                keep.append((x, st, px, mask, absolute, sel))
+            cpa.batch_run(ctx, jobs)
+            for job, (x, st, px, mask, absolute, sel) in zip(jobs, keep):
+                img = px.to_host(np.zeros(x[0].shape, np.uint8))
+                m = mask.to_host(np.zeros(x[0].shape, np.uint8))
+                thr = O.threshold(absolute, 0.05 if absolute else 2.0, 0.2 if absolute else 4.0, 1)
+                oimg, omask, _ = O.outlier(x, thr, 0, int(sel))
+                assert np.array_equal(img, oimg), (pick, x.shape[0], absolute, sel)
+                assert np.array_equal(m, omask), (pick, x.shape[0], absolute, sel)
+                st.close()
+                px.free()
+                mask.free()
+    finally:
+        cpa.tile_shapes_set(ctx, 1, 3, [None] * 6)
+
+
 @pytest.mark.parametrize("bps", [1, 2])
 def test_autotune_keeps_results(ctx, bps):
     shapes = cpa.autotune(ctx, bps, 3)
