@@ -268,6 +268,26 @@ def test_outlier_every_candidate_tile_shape(ctx, bits, c):
         cpa.tile_shapes_set(ctx, bits // 8, c, [None] * 6)
 
 
+@pytest.mark.parametrize("bits", [8, 16])
+def test_every_candidate_tile_shape_all_modes(ctx, bits):
+    """All background x selection modes, absolute and relative, under both candidate shapes
+    of the 64- and 128-frame buckets."""
+    hi = (1 << bits) - 1
+    try:
+        for pick in (0, 1):
+            shapes = [cands[min(pick, len(cands) - 1)] for cands in CANDIDATES[bits]]
+            cpa.tile_shapes_set(ctx, bits // 8, 3, shapes)
+            for n in (64, 128):
+                x = make_stack(2700 + n + bits + pick, n, 3, 21, 3, bits=bits, outlier_frac=0.1)
+                x[: (n - 1) // 2, 1, 2:6, :] = hi // 40
+                for bg, sel in MODES:
+                    for absolute, lo, thr_hi in ((True, 0.05, 0.2), (False, 2.0, 4.0)):
+                        gpu, ref = run_both(ctx, x, absolute, lo, thr_hi, bg, sel)
+                        assert_same(gpu, ref, f"bits={bits} n={n} pick={pick} bg={bg} sel={sel} abs={absolute}")
+    finally:
+        cpa.tile_shapes_set(ctx, bits // 8, 3, [None] * 6)
+
+
 def test_tile_shapes_set_rejects_non_candidates(ctx):
     with pytest.raises(cpa.ChronoError) as e:
         cpa.tile_shapes_set(ctx, 1, 3, [None, (8, 64, 1), None, None, None, None])
