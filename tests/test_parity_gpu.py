@@ -288,6 +288,93 @@ def test_every_candidate_tile_shape_all_modes(ctx, bits):
         cpa.tile_shapes_set(ctx, bits // 8, 3, [None] * 6)
 
 
+def test_random_configurations(ctx):
+    """300 configurations drawn from a fixed seed: sample width, 1-4 channels, 1-300 frames,
+    image shape, threshold kind and limits, background and selection mode, weights, fade,
+    frame selection, crop and pinned tile shape, each against the oracle."""
+    rng = np.random.default_rng(20261203)
+    for case in range(300):
+        bits = int(rng.choice([8, 16]))
+        c = int(rng.integers(1, 5))
+        n = int(rng.choice([int(rng.integers(1, 40)), int(rng.integers(40, 300))]))
+        w, h = int(rng.integers(8, 60)), int(rng.integers(4, 7))
+        x = make_stack(int(rng.integers(1 << 30)), n, h, w, c, bits=bits, outlier_frac=float(rng.uniform(0, 0.5)))
+        slots = None
+        if n > 4 and rng.random() < 0.3:
+            slots = sorted(rng.choice(n, int(rng.integers(3, n)), replace=False).tolist())
+        m = n if slots is None else len(slots)
+        absolute = m < 3 or rng.random() < 0.5
+        lo = float(rng.uniform(0.01, 0.3)) if absolute else float(rng.uniform(0.5, 3.0))
+        hi = lo + float(rng.uniform(0, 0.3 if absolute else 2.0))
+        bg, sel = int(rng.integers(0, 4)), int(rng.integers(0, 6))
+        weights = tuple(float(v) for v in rng.choice([1.0, 0.5, 2.0, 0.0, -1.0], 4))
+        if all(v == 0.0 for v in weights[:c]):
+            weights = (1.0,) + weights[1:]
+        fade = None
+        if rng.random() < 0.4:
+            keys = sorted(rng.choice(max(n, 4), int(rng.integers(2, 5)), replace=False).tolist())
+            fade = (bool(rng.random() < 0.5), bool(rng.random() < 0.5),
+                    [(int(k), float(rng.choice([0.0, 0.25, 0.5, 1.0, float(rng.random())]))) for k in keys])
+        crop = None
+        if rng.random() < 0.3:
+            offs = rng.integers(-2, 3, (m, 2)).tolist()
+            offs[0] = [0, 0]
+            cr = O.crop_create(offs, w, h)
+            if cr is not None and cr[1] > 0 and cr[2] > 0:  # +-2 on 4 rows can leave nothing
+                crop = cr
+        if c >= 3 and rng.random() < 0.5:
+            cands = CANDIDATES[bits]
+            cpa.tile_shapes_set(ctx, bits // 8, c, [cs[int(rng.integers(0, len(cs)))] for cs in cands])
+        try:
+            gpu, ref = run_both(ctx, x, absolute, lo, hi, bg, sel, weights=weights, fade=fade, slots=slots,
+                                crop=crop, seed=int(rng.integers(1 << 31)))
+            assert_same(gpu, ref, f"case {case}: bits={bits} c={c} n={n} {w}x{h} abs={absolute} lo={lo} "
+                                  f"hi={hi} bg={bg} sel={sel} w={weights} fade={fade} slots={slots} "
+                                  f"crop={crop is not None}")
+        finally:
+            if c >= 3:
+                cpa.tile_shapes_set(ctx, bits // 8, c, [None] * 6)
+
+
+def test_random_simple_configurations(ctx):
+    """120 darker/lighter configurations from a fixed seed: sample width, 1-4 channels,
+    1-200 frames, width, weights, fade, frame selection and crop, against the oracle."""
+    rng = np.random.default_rng(20261204)
+    for case in range(120):
+        bits = int(rng.choice([8, 16]))
+        c = int(rng.integers(1, 5))
+        n = int(rng.integers(1, 200))
+        w, h = int(rng.integers(2, 70)), int(rng.integers(1, 7))  # make_stack writes two tie pixels
+        x = make_stack(int(rng.integers(1 << 30)), n, h, w, c, bits=bits, outlier_frac=float(rng.uniform(0, 0.5)))
+        st = cpa.Stack.from_array(ctx, x)
+        darker = bool(rng.random() < 0.5)
+        weights = tuple(float(v) for v in rng.choice([1.0, 0.5, 2.0, 0.0, -1.0], 4))
+        slots = None
+        if n > 3 and rng.random() < 0.3:
+            slots = sorted(rng.choice(n, int(rng.integers(2, n)), replace=False).tolist())
+        m = n if slots is None else len(slots)
+        fade, cfade, ofade = None, None, None
+        if rng.random() < 0.4:
+            keys = sorted(rng.choice(max(n, 4), int(rng.integers(2, 5)), replace=False).tolist())
+            fade = (bool(rng.random() < 0.5), bool(rng.random() < 0.5),
+                    [(int(k), float(rng.choice([0.0, 0.25, 0.5, 1.0, float(rng.random())]))) for k in keys])
+            cfade = cpa.Fade(*fade)
+            ofade = O.Fade(fade[0], fade[1], [f for f, _ in fade[2]], [v for _, v in fade[2]])
+        crop = None
+        if w > 5 and h > 5 and rng.random() < 0.3:
+            offs = rng.integers(-2, 3, (m, 2)).tolist()
+            offs[0] = [0, 0]
+            cr = O.crop_create(offs, w, h)
+            if cr is not None and cr[1] > 0 and cr[2] > 0:
+                crop = cr
+        img = cpa.SimpleProcessor(weights, cfade, darker).process(st, image_indices=slots, crop=crop)
+        ref = O.simple(x, weights, ofade, darker, slots=slots, crop_xy=None if crop is None else crop[0],
+                       out_w=None if crop is None else crop[1], out_h=None if crop is None else crop[2])
+        assert np.array_equal(img, ref), f"case {case}: bits={bits} c={c} n={n} {w}x{h} darker={darker} " \
+                                         f"w={weights} fade={fade} slots={slots} crop={crop is not None}"
+        st.close()
+
+
 def test_tile_shapes_set_rejects_non_candidates(ctx):
     with pytest.raises(cpa.ChronoError) as e:
         cpa.tile_shapes_set(ctx, 1, 3, [None, (8, 64, 1), None, None, None, None])
