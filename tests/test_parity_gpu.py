@@ -587,6 +587,31 @@ def test_shake_frame_selection_single_frame_and_no_anchors(ctx):
     assert off[1:].tolist() == [[-4, -4]] * (len(shifts) - 1)
 
 
+def test_shake_random_configurations(ctx):
+    """60 shake searches from a fixed seed: sample width, 1-4 channels, 2-9 frames (some
+    through a frame selection), 1-4 anchors anywhere they fit, r in 0..14, s in 0..11."""
+    rng = np.random.default_rng(20261205)
+    for case in range(60):
+        bits = int(rng.choice([8, 16]))
+        c = int(rng.integers(1, 5))
+        r, s = int(rng.integers(0, 15)), int(rng.integers(0, 12))
+        n = int(rng.integers(2, 10))
+        h, w = 2 * (r + s) + int(rng.integers(1, 20)), 2 * (r + s) + int(rng.integers(1, 30))
+        hi = (1 << bits) - 1
+        x = rng.integers(0, hi + 1, (n, h, w, c)).astype(np.uint8 if bits == 8 else np.uint16)
+        anchors = [(int(rng.integers(r + s, w - r - s)), int(rng.integers(r + s, h - r - s)))
+                   for _ in range(int(rng.integers(1, 5)))]
+        slots = None
+        if rng.random() < 0.3:
+            slots = rng.permutation(n)[: int(rng.integers(1, n + 1))].tolist()
+        st = cpa.Stack.from_array(ctx, x)
+        off = cpa.ShakeAnalyzer().analyze(st, anchors, r, s, frames=slots)
+        ref = O.shake(x, anchors, r, s, slots=slots)
+        assert np.array_equal(off, ref), f"case {case}: bits={bits} c={c} n={n} {w}x{h} r={r} s={s} " \
+                                         f"anchors={anchors} slots={slots}"
+        st.close()
+
+
 def test_shake_anchor_errors(ctx):
     x = textured(2, 30, 30, [(0, 0), (1, 1)])
     st = cpa.Stack.from_array(ctx, x)
