@@ -157,6 +157,52 @@ def test_outlier_host_bands_equal_resident(ctx, bits, c):
     st.close()
 
 
+def test_host_bands_random_configurations(ctx):
+    """40 out-of-core composites from a fixed seed (band heights from 1 row to the whole
+    image, crops, frame selections, outlier and darker/lighter modes, 1-4 channels) equal the
+    resident composite, statistics included."""
+    rng = np.random.default_rng(20261206)
+    for case in range(40):
+        bits = int(rng.choice([8, 16]))
+        c = int(rng.integers(1, 5))
+        n = int(rng.integers(3, 60))
+        h, w = int(rng.integers(6, 24)), int(rng.integers(6, 40))
+        x = make_stack(int(rng.integers(1 << 30)), n, h, w, c, bits=bits, outlier_frac=float(rng.uniform(0, 0.4)))
+        st = cpa.Stack.from_array(ctx, x)
+        slots = None
+        if rng.random() < 0.3:
+            slots = sorted(rng.choice(n, int(rng.integers(3, n + 1)), replace=False).tolist())
+        m = n if slots is None else len(slots)
+        crop, halo = None, 0
+        if rng.random() < 0.5:
+            offs = rng.integers(-2, 3, (m, 2)).tolist()
+            offs[0] = [0, 0]
+            cr = O.crop_create(offs, w, h)
+            if cr is not None and cr[1] > 0 and cr[2] > 0:
+                crop = cr
+                halo = int(cr[0][:, 1].max() - cr[0][:, 1].min())
+        row = w * c * (bits // 8)
+        band_rows = int(rng.integers(1, h + 3))
+        band_bytes = (band_rows + halo) * row * m + 256
+        if rng.random() < 0.7:
+            absolute = bool(rng.random() < 0.5)
+            thr = cpa.Threshold.abs(0.05, 0.2) if absolute else cpa.Threshold.rel(2.0, 4.0)
+            proc = cpa.OutlierProcessor(thr, int(rng.integers(0, 4)), int(rng.integers(0, 6)),
+                                        seed=int(rng.integers(1 << 31)))
+            ref = proc.process(st, image_indices=slots, crop=crop, with_stats=True)
+            ref_warn = proc.warnings
+            got = proc.process_host(ctx, x, image_indices=slots, crop=crop, with_stats=True, band_bytes=band_bytes)
+            for a, b, what in zip(got, ref, ("image", "mask", "stats")):
+                assert np.array_equal(a, b), f"case {case}: {what} bits={bits} c={c} n={n} {w}x{h} band={band_rows}"
+            assert proc.warnings == ref_warn, f"case {case}: warnings"
+        else:
+            proc = cpa.SimpleProcessor(darker=bool(rng.random() < 0.5))
+            ref = proc.process(st, image_indices=slots, crop=crop)
+            got = proc.process_host(ctx, x, image_indices=slots, crop=crop, band_bytes=band_bytes)
+            assert np.array_equal(got, ref), f"case {case}: simple bits={bits} c={c} n={n} {w}x{h} band={band_rows}"
+        st.close()
+
+
 @pytest.mark.parametrize("bits,c", [(8, 3), (16, 4)])
 def test_simple_host_bands_equal_resident(ctx, bits, c):
     """Darker / lighter over host frames in row bands equal the resident composite, with crops,
