@@ -143,6 +143,58 @@ def test_batch_with_every_candidate_shape_against_oracle(ctx):
         cpa.tile_shapes_set(ctx, 1, 3, [None] * 6)
 
 
+def test_batch_random_mixes(ctx):
+    """Five batches of 12 jobs drawn from a fixed seed (outlier and darker/lighter, u8/u16,
+    1-4 channels, 1-300 frames, 1x1 to 150x40 images, every mode), graph-replayed twice, each
+    job against the oracle."""
+    rng = np.random.default_rng(20261207)
+    for batch_no in range(5):
+        jobs, keep = [], []
+        for j in range(12):
+            bps = int(rng.choice([1, 2]))
+            c = int(rng.integers(1, 5))
+            n = int(rng.choice([int(rng.integers(1, 40)), int(rng.integers(40, 300))]))
+            w, h = int(rng.integers(1, 150)), int(rng.integers(1, 40))
+            hi = (1 << (8 * bps)) - 1
+            x = rng.integers(hi // 4, 3 * hi // 4, (1, h, w, c)) + rng.integers(-hi // 40, hi // 40 + 1, (n, h, w, c))
+            hit = rng.random((n, h, w)) < 0.15
+            x = np.where(hit[..., None], rng.integers(0, hi + 1, (n, h, w, c)), x)
+            x = np.clip(x, 0, hi).astype(np.uint8 if bps == 1 else np.uint16)
+            st = cpa.Stack.from_array(ctx, x)
+            px, mask = H.DeviceBuffer(x[0].nbytes), H.DeviceBuffer(x[0].size)
+            mode = int(rng.choice([cpa.CP_MODE_OUTLIER, cpa.CP_MODE_OUTLIER, cpa.CP_MODE_DARKER, cpa.CP_MODE_LIGHTER]))
+            absolute = n < 3 or bool(rng.random() < 0.5)
+            bg, sel = int(rng.integers(0, 4)), int(rng.integers(0, 6))
+            if mode == cpa.CP_MODE_OUTLIER:
+                thr = cpa.Threshold.abs(0.05, 0.2) if absolute else cpa.Threshold.rel(2.0, 4.0)
+                proc = cpa.OutlierProcessor(thr, bg, sel, seed=7)
+                jobs.append(cpa.Job(mode, st, px.ptr, mask.ptr, processor=proc))
+            else:
+                jobs.append(cpa.Job(mode, st, px.ptr))
+            keep.append((x, st, px, mask, mode, absolute, bg, sel))
+        batch = cpa.Batch(ctx, jobs, graph=True)
+        for rep in range(2):
+            for k in keep:
+                k[2].from_host(np.zeros(k[2].nbytes, np.uint8))
+            batch.launch()
+            ctx.synchronize()
+            for j, (x, st, px, mask, mode, absolute, bg, sel) in enumerate(keep):
+                img = px.to_host(np.zeros(x[0].shape, x.dtype))
+                if mode == cpa.CP_MODE_OUTLIER:
+                    thr = O.threshold(absolute, 0.05 if absolute else 2.0, 0.2 if absolute else 4.0, x.dtype.itemsize)
+                    oimg, omask, _ = O.outlier(x, thr, bg, sel, seed=7)
+                    m = mask.to_host(np.zeros(x[0].shape, np.uint8))
+                    assert np.array_equal(img, oimg) and np.array_equal(m, omask), (batch_no, j, rep, x.shape, bg, sel)
+                else:
+                    ref = O.simple(x, darker=mode == cpa.CP_MODE_DARKER)
+                    assert np.array_equal(img, ref), (batch_no, j, rep, x.shape, mode)
+        batch.close()
+        for x, st, px, mask, *_ in keep:
+            st.close()
+            px.free()
+            mask.free()
+
+
 @pytest.mark.parametrize("bps", [1, 2])
 def test_autotune_keeps_results(ctx, bps):
     shapes = cpa.autotune(ctx, bps, 3)
