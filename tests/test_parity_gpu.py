@@ -382,6 +382,27 @@ def test_random_configurations(ctx):
                 cpa.tile_shapes_set(ctx, bits // 8, c, [None] * 6)
 
 
+def test_random_two_pixel_tiles_with_crops(ctx):
+    """24 configurations of the two-pixel 16-bit path from a fixed seed: 100-256 frames,
+    several tiles per row, per-frame crop origins of up to +-8 pixels (every row alignment of
+    the pair loads), 3 and 4 channels, absolute thresholds in the modes the default workload
+    uses and beyond."""
+    rng = np.random.default_rng(20261208)
+    for case in range(24):
+        c = int(rng.choice([3, 4]))
+        n = int(rng.integers(100, 257))
+        h, w = int(rng.integers(18, 26)), int(rng.integers(60, 300))
+        x = make_stack(int(rng.integers(1 << 30)), n, h, w, c, bits=16, outlier_frac=float(rng.uniform(0, 0.2)))
+        offs = rng.integers(-8, 9, (n, 2)).tolist()
+        offs[0] = [0, 0]
+        crop = O.crop_create(offs, w, h)
+        bg, sel = int(rng.choice([0, 0, 1, 2, 3])), int(rng.choice([4, 4, 0, 1, 2, 3, 5]))
+        lo = float(rng.uniform(0.02, 0.1))
+        gpu, ref = run_both(ctx, x, True, lo, lo + float(rng.uniform(0, 0.2)), bg, sel, crop=crop,
+                            seed=int(rng.integers(1 << 31)))
+        assert_same(gpu, ref, f"case {case}: c={c} n={n} {w}x{h} bg={bg} sel={sel}")
+
+
 def test_random_simple_configurations(ctx):
     """120 darker/lighter configurations from a fixed seed: sample width, 1-4 channels,
     1-200 frames, width, weights, fade, frame selection and crop, against the oracle."""
