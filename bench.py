@@ -74,6 +74,19 @@ WORKLOADS = {
                    disk_val_lo=2000, disk_val_hi=4000, shake_max=8, texture_amp=3000,
                    disk_y_lo=540, disk_y_hi=1620),
         shake=None, mode="darker"),
+    # a gallery-sized photo: about 1200 FHD frames per composite
+    "outlier_1080p_rgb8_1200f": dict(
+        scene=dict(seed=6, width=1920, height=1080, channels=3, bytes_per_sample=1, n_frames=1200,
+                   base_lo=40, base_hi=200, sigma=3.0, n_disks=3, disk_r_min=10, disk_r_max=40,
+                   disk_val_lo=15, disk_val_hi=25, blob_permille=20, blob_r=30, blob_val=230),
+        shake=None, mode="outlier", threshold=("abs", 0.05, 0.2), background="first", outlier="extreme"),
+    # the same stack with `--sample 64`: medians over 64 seeded frame positions, distances over all
+    "outlier_1080p_rgb8_1200f_s64": dict(
+        scene=dict(seed=6, width=1920, height=1080, channels=3, bytes_per_sample=1, n_frames=1200,
+                   base_lo=40, base_hi=200, sigma=3.0, n_disks=3, disk_r_min=10, disk_r_max=40,
+                   disk_val_lo=15, disk_val_hi=25, blob_permille=20, blob_r=30, blob_val=230),
+        shake=None, mode="outlier", threshold=("abs", 0.05, 0.2), background="first", outlier="extreme",
+        sample=64),
     "outlier_640x480_rgb8_16f": dict(
         scene=dict(seed=0, width=640, height=480, channels=3, bytes_per_sample=1, n_frames=16,
                    base_lo=40, base_hi=200, sigma=3.0, n_disks=1, disk_r_min=20, disk_r_max=20,
@@ -317,8 +330,13 @@ def main():
     if wl["mode"] == "outlier":
         kind, lo, hi = wl["threshold"]
         thr = cpa.Threshold(kind == "abs", lo, hi)
+        sample_pos = None
+        if wl.get("sample"):
+            # `--sample k` (src/chrono.rs:151-163): k distinct frame positions, sorted, seeded here
+            sample_pos = np.sort(np.random.default_rng(7).choice(n, wl["sample"], replace=False))
         proc = cpa.OutlierProcessor(thr, cpa.BackgroundMode.from_str(wl["background"]),
-                                    cpa.OutlierSelectionMode.from_str(wl["outlier"]))
+                                    cpa.OutlierSelectionMode.from_str(wl["outlier"]),
+                                    sample_indices=sample_pos)
         params = proc.params(bps)
         outs = cpa.cp_outputs(out_px.ptr, out_mask.ptr, None, None)
 
@@ -367,7 +385,7 @@ def main():
         if wl["mode"] == "outlier":
             othr = O.threshold(thr.absolute, thr.min, thr.max, bps)
             bgm, selm = int(proc.background), int(proc.outlier)
-            kw = dict(crop_xy=None if crop is None else crop[0], out_w=ow, out_h=rows)
+            kw = dict(crop_xy=None if crop is None else crop[0], out_w=ow, out_h=rows, sample_pos=sample_pos)
             t0 = time.time()
             ref_img, _, _ = O.outlier(band, othr, bgm, selm, threads=threads, y0=0, y1=rows, **kw)
             cpu_s = time.time() - t0
@@ -421,6 +439,7 @@ def main():
                    "out_width": ow, "out_height": oh, "mode": wl["mode"],
                    **({"threshold": "/".join(map(str, wl["threshold"])), "background": wl["background"],
                        "outlier": wl["outlier"]} if wl["mode"] == "outlier" else {}),
+                   **({"sample": wl["sample"]} if wl.get("sample") else {}),
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved_gbs / HBM_PEAK_GBS,

@@ -9,6 +9,7 @@ semantics: no FMA contraction, correctly rounded division and square root, no de
 flushing, and no packed-f32 instruction selection.
 """
 import os
+import re
 import subprocess
 import sys
 import concurrent.futures as cf
@@ -36,8 +37,27 @@ LIBS = {
 }
 
 
-def _compile(src):
+def _deps(src, seen=None):
+    """The source and every local header it includes, transitively."""
+    seen = set() if seen is None else seen
+    path = os.path.join(CSRC, src)
+    if path in seen or not os.path.exists(path):
+        return seen
+    seen.add(path)
+    with open(path) as f:
+        for line in f:
+            m = re.match(r'\s*#include\s+"([^"]+)"', line)
+            if m:
+                _deps(os.path.normpath(os.path.join(os.path.dirname(src), m.group(1))), seen)
+    return seen
+
+
+def _compile(src, changed_only=False):
     obj = os.path.join(OUT, "obj", os.path.basename(src) + ".o")
+    if changed_only and os.path.exists(obj):
+        t = os.path.getmtime(obj)
+        if all(os.path.getmtime(d) <= t for d in _deps(src)):
+            return obj
     cmd = [HIPCC, *FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     lines = [l for l in (r.stdout + r.stderr).splitlines()
@@ -47,11 +67,13 @@ def _compile(src):
     return obj
 
 
-def build(verbose=False):
+def build(verbose=False, changed_only=False):
+    """Compiles every translation unit (changed_only: only those whose source or local
+    headers are newer than their object, for development) and links the libraries."""
     os.makedirs(os.path.join(OUT, "obj"), exist_ok=True)
     srcs = sorted({s for v in LIBS.values() for s in v})
     with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
-        objs = dict(zip(srcs, ex.map(_compile, srcs)))
+        objs = dict(zip(srcs, ex.map(lambda s: _compile(s, changed_only), srcs)))
     for lib, parts in LIBS.items():
         out = os.path.join(OUT, lib)
         cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", out + ".tmp",
@@ -86,5 +108,5 @@ if __name__ == "__main__":
     if sys.argv[1:] == ["cli"]:
         build_cli(verbose=True)
     else:
-        build(verbose=True)
+        build(verbose=True, changed_only="--changed" in sys.argv[1:])
     sys.exit(0)

@@ -248,7 +248,29 @@ CP_API int cp_ctx_create(int32_t device, uint64_t hbm_budget_bytes, cp_ctx** out
 CP_API int cp_ctx_destroy(cp_ctx* ctx) {
     if (!ctx) return CP_OK;
     (void)hipSetDevice(ctx->device);
+    // Handles still alive: servers first (their workers launch on stacks), then batches, then
+    // stacks.  Each keeps its host shell until its own destroy call, which then only frees it.
+    std::vector<cp_server*> servers;
+    std::vector<cp_batch*> batches;
+    std::vector<cp_stack*> stacks;
+    {
+        std::lock_guard<std::mutex> l(ctx->reg_mu);
+        servers.assign(ctx->servers.begin(), ctx->servers.end());
+        batches.assign(ctx->batches.begin(), ctx->batches.end());
+        stacks.assign(ctx->stacks.begin(), ctx->stacks.end());
+        ctx->servers.clear();
+        ctx->batches.clear();
+        ctx->stacks.clear();
+    }
+    for (cp_server* v : servers) server_orphan(v);
+    for (cp_batch* b : batches) batch_orphan(b);
     (void)hipStreamSynchronize(ctx->stream);
+    for (cp_stack* st : stacks) {
+        if (st->owned && st->dev) (void)hipFree(st->dev);
+        st->dev = nullptr;
+        st->count = st->capacity = 0;
+        st->ctx = nullptr;
+    }
     for (DeviceBuffer* b : {&ctx->out_px, &ctx->out_mask, &ctx->out_stats, &ctx->warn})
         if (b->ptr) (void)hipFree(b->ptr);
     ctx->arena.release();
@@ -292,6 +314,7 @@ CP_API int cp_stack_create(cp_ctx* ctx, uint32_t width, uint32_t height, uint32_
     }
     s->alloc_bytes = bytes;
     s->owned = true;
+    ctx->attach(s);
     *out = s;
     return CP_OK;
 }
@@ -316,12 +339,20 @@ CP_API int cp_stack_wrap(cp_ctx* ctx, void* device_ptr, uint32_t width, uint32_t
     s->dev = (uint8_t*)device_ptr;
     s->alloc_bytes = s->frame_bytes() * n_frames;
     s->owned = false;
+    ctx->attach(s);
     *out = s;
     return CP_OK;
 }
 
+static int check_live(const cp_stack* st) {
+    if (!st) return fail(CP_ERR_INVALID_ARG, "null stack");
+    if (!st->ctx) return fail(CP_ERR_INVALID_ARG, "the stack's context was destroyed");
+    return CP_OK;
+}
+
 CP_API int cp_stack_upload(cp_stack* st, uint32_t slot, const void* host_frames, uint32_t n) {
-    if (!st || (!host_frames && n)) return fail(CP_ERR_INVALID_ARG, "null argument");
+    if (int rc = check_live(st)) return rc;
+    if (!host_frames && n) return fail(CP_ERR_INVALID_ARG, "null argument");
     if ((uint64_t)slot + n > st->capacity)
         return fail(CP_ERR_CAPACITY, "slots [%u,%u) exceed capacity %u", slot, slot + n, st->capacity);
     (void)hipSetDevice(st->ctx->device);
@@ -334,7 +365,7 @@ CP_API int cp_stack_upload(cp_stack* st, uint32_t slot, const void* host_frames,
 }
 
 CP_API int cp_stack_push(cp_stack* st, const void* host_frames, uint32_t n, uint32_t* first_slot) {
-    if (!st) return fail(CP_ERR_INVALID_ARG, "null stack");
+    if (int rc = check_live(st)) return rc;
     uint32_t first = st->count;
     int rc = cp_stack_upload(st, first, host_frames, n);
     if (rc) return rc;
@@ -343,7 +374,8 @@ CP_API int cp_stack_push(cp_stack* st, const void* host_frames, uint32_t n, uint
 }
 
 CP_API int cp_stack_download(const cp_stack* st, uint32_t slot, void* host_frames, uint32_t n) {
-    if (!st || (!host_frames && n)) return fail(CP_ERR_INVALID_ARG, "null argument");
+    if (int rc = check_live(st)) return rc;
+    if (!host_frames && n) return fail(CP_ERR_INVALID_ARG, "null argument");
     if ((uint64_t)slot + n > st->count)
         return fail(CP_ERR_INVALID_ARG, "slots [%u,%u) not loaded (stack holds %u)", slot, slot + n, st->count);
     (void)hipSetDevice(st->ctx->device);
@@ -354,7 +386,7 @@ CP_API int cp_stack_download(const cp_stack* st, uint32_t slot, void* host_frame
 }
 
 CP_API int cp_stack_commit(cp_stack* st, uint32_t count) {
-    if (!st) return fail(CP_ERR_INVALID_ARG, "null stack");
+    if (int rc = check_live(st)) return rc;
     if (count > st->capacity) return fail(CP_ERR_CAPACITY, "count %u exceeds capacity %u", count, st->capacity);
     st->count = count;
     return CP_OK;
@@ -374,6 +406,11 @@ CP_API int cp_stack_info_get(const cp_stack* st, cp_stack_info* out) {
 
 CP_API int cp_stack_destroy(cp_stack* st) {
     if (!st) return CP_OK;
+    if (!st->ctx) {  // its context was destroyed first and already released the memory
+        delete st;
+        return CP_OK;
+    }
+    st->ctx->detach(st);
     if (st->owned && st->dev) {
         (void)hipSetDevice(st->ctx->device);
         (void)hipStreamSynchronize(st->ctx->stream);

@@ -67,14 +67,33 @@ static int run_direct(cp_ctx* ctx, DirectJob& d) {
                      CP_FLAG_DEVICE_OUT);
 }
 
-extern "C" CP_API int cp_batch_destroy(cp_batch* b) {
-    if (!b) return CP_OK;
+// Device resources of a batch (graph, staged tables); the context stream is drained first.
+static void batch_release(cp_batch* b) {
+    (void)hipSetDevice(b->ctx->device);
     (void)hipStreamSynchronize(b->ctx->stream);
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     if (b->d_tables) {
         (void)hipFree(b->d_tables);
         b->ctx->uncharge(b->table_bytes);
+    }
+    b->exec = nullptr;
+    b->graph = nullptr;
+    b->d_tables = nullptr;
+    b->groups.clear();
+    b->direct.clear();
+}
+
+void cp::batch_orphan(cp_batch* b) {
+    batch_release(b);
+    b->ctx = nullptr;
+}
+
+extern "C" CP_API int cp_batch_destroy(cp_batch* b) {
+    if (!b) return CP_OK;
+    if (b->ctx) {
+        b->ctx->detach(b);
+        batch_release(b);
     }
     delete b;
     return CP_OK;
@@ -84,7 +103,14 @@ extern "C" CP_API int cp_batch_create(cp_ctx* ctx, const cp_job* jobs, uint32_t 
                                       cp_batch** out) {
     if (!ctx || !out || (!jobs && n_jobs)) return fail(CP_ERR_INVALID_ARG, "null argument");
     *out = nullptr;
-    std::unique_ptr<cp_batch> b(new cp_batch());
+    // a batch that fails half-way releases what it already holds
+    struct Release {
+        void operator()(cp_batch* p) const {
+            if (p->ctx) batch_release(p);
+            delete p;
+        }
+    };
+    std::unique_ptr<cp_batch, Release> b(new cp_batch());
     b->ctx = ctx;
     for (uint32_t i = 0; i < n_jobs; ++i) {
         const cp_job& j = jobs[i];
@@ -231,12 +257,14 @@ extern "C" CP_API int cp_batch_create(cp_ctx* ctx, const cp_job* jobs, uint32_t 
         b->graph = graph;
         CP_HIP_TRY(hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0));
     }
+    ctx->attach(b.get());
     *out = b.release();
     return CP_OK;
 }
 
 extern "C" CP_API int cp_batch_launch(cp_batch* b) {
     if (!b) return fail(CP_ERR_INVALID_ARG, "null batch");
+    if (!b->ctx) return fail(CP_ERR_INVALID_ARG, "the batch's context was destroyed");
     cp_ctx* ctx = b->ctx;
     {
         std::lock_guard<std::mutex> lock(ctx->mu);
@@ -314,63 +342,78 @@ extern "C" CP_API int cp_autotune(cp_ctx* ctx, uint32_t bps, uint32_t channels, 
     cp_stack* st = nullptr;
     int rc = cp_stack_create(ctx, W, H, channels, bps, NMAX, &st);
     if (rc) return rc;
-    std::lock_guard<std::mutex> lock(ctx->mu);
-    (void)hipSetDevice(ctx->device);
-    hipStream_t s = ctx->stream;
     const uint64_t fe = (uint64_t)W * H * channels;
-    hipLaunchKernelGGL(autotune_fill_kernel, dim3(1024), dim3(256), 0, s, st->dev, bps, fe * NMAX, fe,
-                       bps == 1 ? 255u : 65535u);
-    st->count = NMAX;
-    std::vector<uint64_t> pos(NMAX);
-    for (uint32_t t = 0; t < NMAX; ++t) pos[t] = (uint64_t)t * fe;
+    const uint64_t pos_bytes = 8ull * NMAX, out_bytes = fe * (bps + 1);
     uint64_t* d_pos = nullptr;
     void* d_out = nullptr;
-    hipEvent_t e0, e1;
-    bool ok = hipMalloc((void**)&d_pos, 8 * NMAX) == hipSuccess &&
-              hipMalloc(&d_out, (uint64_t)W * H * channels * (bps + 1)) == hipSuccess &&
-              hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess &&
-              hipMemcpyAsync(d_pos, pos.data(), 8 * NMAX, hipMemcpyHostToDevice, s) == hipSuccess;
-    for (int b = 0; ok && b < kNumBuckets; ++b) {
-        int best = kBucketCand[bps - 1][b][0];
-        if (kBucketCand[bps - 1][b][1] >= 0) {
-            float best_ms = 1e30f;
-            for (int c = 0; c < 2; ++c) {
-                const int shape = kBucketCand[bps - 1][b][c];
-                OutlierK k{};
-                k.stack = st->dev;
-                k.pos_off = d_pos;
-                k.n = kBucketMax[b];
-                k.n_sample = k.n;
-                k.stack_w = W;
-                k.out_w = W;
-                k.out_h = H;
-                k.th = ThresholdK{1, 0.05f * (bps == 1 ? 255.0f : 65535.0f), 0.2f * (bps == 1 ? 255.0f : 65535.0f), 1.0f};
-                k.background = 0;
-                k.outlier = 2;
-                for (int i = 0; i < 4; ++i) k.w[i] = 1.0f;
-                k.fade.is_none = 1;
-                k.out_px = d_out;
-                k.out_mask = (uint8_t*)d_out + (uint64_t)W * H * channels * bps;
-                launch_outlier_fast(bps, channels, k, shape, s);  // warm-up
-                ok = hipEventRecord(e0, s) == hipSuccess;
-                for (int r = 0; r < 3; ++r) launch_outlier_fast(bps, channels, k, shape, s);
-                ok = ok && hipEventRecord(e1, s) == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
-                float ms = 0.0f;
-                ok = ok && hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
-                if (ok && ms < best_ms) { best_ms = ms; best = shape; }
-            }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    bool charged = false, ok = true;
+    {
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        (void)hipSetDevice(ctx->device);
+        hipStream_t s = ctx->stream;
+        rc = ctx->charge(pos_bytes + out_bytes);
+        charged = rc == CP_OK;
+        ok = charged;
+        if (ok) {
+            hipLaunchKernelGGL(autotune_fill_kernel, dim3(1024), dim3(256), 0, s, st->dev, bps, fe * NMAX, fe,
+                               bps == 1 ? 255u : 65535u);
+            st->count = NMAX;
         }
-        ctx->tuned[bps - 1][channels - 1][b] = (int8_t)best;
-        if (shapes) shapes[b] = (kShapes[best].X << 16) | (kShapes[best].G << 8) | kShapes[best].F;
+        std::vector<uint64_t> pos(NMAX);
+        for (uint32_t t = 0; t < NMAX; ++t) pos[t] = (uint64_t)t * fe;
+        ok = ok && hipMalloc((void**)&d_pos, pos_bytes) == hipSuccess;
+        ok = ok && hipMalloc(&d_out, out_bytes) == hipSuccess;
+        ok = ok && hipEventCreate(&e0) == hipSuccess;
+        ok = ok && hipEventCreate(&e1) == hipSuccess;
+        ok = ok && hipMemcpyAsync(d_pos, pos.data(), pos_bytes, hipMemcpyHostToDevice, s) == hipSuccess;
+        for (int b = 0; ok && b < kNumBuckets; ++b) {
+            int best = kBucketCand[bps - 1][b][0];
+            if (kBucketCand[bps - 1][b][1] >= 0) {
+                float best_ms = 1e30f;
+                for (int c = 0; ok && c < 2; ++c) {
+                    const int shape = kBucketCand[bps - 1][b][c];
+                    OutlierK k{};
+                    k.stack = st->dev;
+                    k.pos_off = d_pos;
+                    k.n = kBucketMax[b];
+                    k.n_sample = k.n;
+                    k.stack_w = W;
+                    k.out_w = W;
+                    k.out_h = H;
+                    const float range = bps == 1 ? 255.0f : 65535.0f;
+                    k.th = ThresholdK{1, 0.05f * range, 0.2f * range, 1.0f};
+                    k.background = 0;
+                    k.outlier = 2;
+                    for (int i = 0; i < 4; ++i) k.w[i] = 1.0f;
+                    k.fade.is_none = 1;
+                    k.out_px = d_out;
+                    k.out_mask = (uint8_t*)d_out + fe * bps;
+                    launch_outlier_fast(bps, channels, k, shape, s);  // warm-up
+                    ok = hipEventRecord(e0, s) == hipSuccess;
+                    for (int r = 0; r < 3; ++r) launch_outlier_fast(bps, channels, k, shape, s);
+                    ok = ok && hipEventRecord(e1, s) == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
+                    float ms = 0.0f;
+                    ok = ok && hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
+                    if (ok && ms < best_ms) {
+                        best_ms = ms;
+                        best = shape;
+                    }
+                }
+            }
+            if (!ok) break;
+            ctx->tuned[bps - 1][channels - 1][b] = (int8_t)best;
+            if (shapes) shapes[b] = (kShapes[best].X << 16) | (kShapes[best].G << 8) | kShapes[best].F;
+        }
+        (void)hipStreamSynchronize(s);
+        if (d_pos) (void)hipFree(d_pos);
+        if (d_out) (void)hipFree(d_out);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        if (charged) ctx->uncharge(pos_bytes + out_bytes);
     }
-    (void)hipStreamSynchronize(s);
-    if (d_pos) (void)hipFree(d_pos);
-    if (d_out) (void)hipFree(d_out);
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    ctx->mu.unlock();
-    cp_stack_destroy(st);
-    ctx->mu.lock();
+    cp_stack_destroy(st);  // takes no context lock of its own: outside the scope above for clarity
+    if (!charged) return rc;
     if (!ok) return fail(CP_ERR_HIP, "autotune failed: %s", hipGetErrorString(hipGetLastError()));
     return CP_OK;
 }

@@ -6,6 +6,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <unordered_set>
 #include <vector>
 
 #include "../../include/chrono_hip.h"
@@ -44,6 +45,20 @@ struct DeviceBuffer {
 }  // namespace cp
 
 struct cp_ctx {
+    // Handles created on this context.  cp_ctx_destroy releases their device resources first
+    // and leaves each as an inert shell (ctx == nullptr) whose own destroy call only frees it,
+    // so a caller may destroy the context before its stacks, batches or servers.
+    std::mutex reg_mu;
+    std::unordered_set<cp_stack*> stacks;
+    std::unordered_set<cp_batch*> batches;
+    std::unordered_set<cp_server*> servers;
+    void attach(cp_stack* s) { std::lock_guard<std::mutex> l(reg_mu); stacks.insert(s); }
+    void attach(cp_batch* b) { std::lock_guard<std::mutex> l(reg_mu); batches.insert(b); }
+    void attach(cp_server* v) { std::lock_guard<std::mutex> l(reg_mu); servers.insert(v); }
+    void detach(cp_stack* s) { std::lock_guard<std::mutex> l(reg_mu); stacks.erase(s); }
+    void detach(cp_batch* b) { std::lock_guard<std::mutex> l(reg_mu); batches.erase(b); }
+    void detach(cp_server* v) { std::lock_guard<std::mutex> l(reg_mu); servers.erase(v); }
+
     int device = 0;
     hipStream_t stream = nullptr;
     uint64_t budget = 0;
@@ -65,7 +80,7 @@ struct cp_ctx {
 constexpr uint64_t kStackSlack = 256;
 
 struct cp_stack {
-    cp_ctx* ctx = nullptr;
+    cp_ctx* ctx = nullptr;  // nullptr once the context was destroyed (an inert shell)
     uint32_t width = 0, height = 0, channels = 0, bps = 0;
     uint32_t capacity = 0, count = 0;
     uint8_t* dev = nullptr;
@@ -103,5 +118,10 @@ FadeK fade_to_device(cp_ctx* ctx, const cp_fade* f, int* rc);
 int launch_outlier(cp_ctx* ctx, const cp_stack* st, const OutlierK& k, uint64_t max_pos, uint32_t flags,
                    hipStream_t s);
 int launch_simple(cp_ctx* ctx, const cp_stack* st, const SimpleK& k, hipStream_t s);
+
+// Release a handle's device resources when its context is destroyed first (batch.hip,
+// server.hip); the handle stays allocated, with ctx == nullptr, until its own destroy call.
+void batch_orphan(cp_batch* b);
+void server_orphan(cp_server* v);
 
 }  // namespace cp

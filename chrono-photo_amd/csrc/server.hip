@@ -95,41 +95,53 @@ uint64_t request_bytes(const cp_request& r, uint32_t* out_w, uint32_t* out_h) {
     *out_w = r.job.frames.out_width ? r.job.frames.out_width : W;
     *out_h = r.job.frames.out_height ? r.job.frames.out_height : H;
     uint64_t bytes = 0;
-    if (!st) bytes += (uint64_t)W * H * C * B * r.n_frames;
+    if (!st) bytes += (uint64_t)W * H * C * B * r.n_frames + kStackSlack;
     if (!r.job.out.px) bytes += (uint64_t)(*out_w) * (*out_h) * C * B;
     if (!r.job.out.mask && r.host_out_mask) bytes += (uint64_t)(*out_w) * (*out_h) * C;
     return bytes;
 }
 
+bool fade_ok(const cp_fade& f) { return f.is_none || (f.len > 0 && f.values != nullptr); }
+
 }  // namespace
 
+// Locking: `mu` guards what submitters and waiters share with the worker -- the arrival
+// queue, the ticket map, each request's status and the published statistics.  Everything
+// else (slots, uploads in transit, ready list, admitted bytes) belongs to the worker thread,
+// which stages parameters, launches and polls events without holding `mu`.
 struct cp_server {
-    cp_ctx* ctx = nullptr;
+    cp_ctx* ctx = nullptr;  // nullptr once the context was destroyed (an inert shell)
     uint64_t budget = 0;
     std::vector<Slot> slots;
     hipStream_t up_stream = nullptr;
-    std::deque<Request*> uploading, ready;  // admitted: frames in transit / waiting for a slot
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::deque<Request*> queue;
     std::unordered_map<uint64_t, std::unique_ptr<Request>> all;
-    uint64_t used = 0, next = 1;
-    bool stop = false;
+    uint64_t next = 1;
+    bool stop = false, stopped = false;
     std::thread worker;
     cp_server_stats stats{};
+    // worker-private
+    std::deque<Request*> uploading, ready;  // admitted: frames in transit / waiting for a slot
+    std::vector<std::pair<Request*, int>> finished;  // completed since the last locked phase
+    uint64_t used = 0;
+    cp_server_stats wstats{};               // the worker's counters, published under `mu`
 
     int upload(Request* r);
     int start(Request* r, Slot& slot);
-    void retire(Slot& slot, int status);
-    void finish(Request* r, int status);
+    void release_slot(Slot& slot, int status);
+    void release_request(Request* r, int status);
     void loop();
+    void shutdown();
 };
 
-// Allocates a request's frames and starts their upload on the upload stream.
+// Allocates a request's frames (with the slack the two-pixel tiles read past the last
+// frame) and starts their upload on the upload stream.
 int cp_server::upload(Request* r) {
     const cp_request& q = r->req;
     const uint64_t bytes = (uint64_t)q.width * q.height * q.channels * q.bytes_per_sample * q.n_frames;
-    if (hipMallocAsync(&r->own_frames, bytes, up_stream) != hipSuccess)
+    if (hipMallocAsync(&r->own_frames, bytes + kStackSlack, up_stream) != hipSuccess)
         return fail(CP_ERR_HIP, "server: frame allocation of %llu bytes failed", (unsigned long long)bytes);
     if (hipMemcpyAsync(r->own_frames, q.host_frames, bytes, hipMemcpyHostToDevice, up_stream) != hipSuccess)
         return fail(CP_ERR_HIP, "server: frame upload failed");
@@ -153,6 +165,7 @@ int cp_server::start(Request* r, Slot& slot) {
         o.bps = q.bytes_per_sample;
         o.capacity = o.count = q.n_frames;
         o.dev = (uint8_t*)r->own_frames;
+        o.alloc_bytes = o.frame_bytes() * q.n_frames + kStackSlack;
         o.owned = false;
         st = &o;
     }
@@ -180,7 +193,9 @@ int cp_server::start(Request* r, Slot& slot) {
             k.sample_mask = (const uint32_t*)slot.arena.put(prep.sample_mask.data(), 4 * prep.sample_mask.size(), s);
         if (!prep.fade_values.empty())
             k.fade.values = (const float*)slot.arena.put(prep.fade_values.data(), 4 * prep.fade_values.size(), s);
-        if (!k.pos_off) return fail(CP_ERR_HIP, "server: parameter staging failed");
+        if (!k.pos_off || (!prep.sample_mask.empty() && !k.sample_mask) ||
+            (!prep.fade_values.empty() && !k.fade.values))
+            return fail(CP_ERR_HIP, "server: parameter staging failed");
         k.out_px = px;
         k.out_mask = mask;
         k.warnings = (unsigned long long*)j.out.warnings;
@@ -204,28 +219,30 @@ int cp_server::start(Request* r, Slot& slot) {
         k.darker = j.mode == CP_MODE_DARKER;
         for (int i = 0; i < 4; ++i) k.w[i] = j.weights[i];
         k.fade.is_none = j.fade.is_none ? 1 : 0;
-        if (!j.fade.is_none) {
+        if (!j.fade.is_none) {  // cp_server_submit checked the table
             k.fade.repeat = j.fade.repeat;
             k.fade.absolute = j.fade.absolute;
             k.fade.offset = j.fade.offset;
             k.fade.len = j.fade.len;
             k.fade.values = (const float*)slot.arena.put(j.fade.values, 4 * j.fade.len, s);
+            if (!k.fade.values) return fail(CP_ERR_HIP, "server: parameter staging failed");
         }
         k.out_px = px;
         if ((rc = launch_simple(ctx, st, k, s))) return rc;
         r->pixel_samples = (uint64_t)k.n * k.out_w * k.out_h;
     }
     const bool by_kernel = !uploading.empty();
-    if (q.host_out_px && download(q.host_out_px, px, px_bytes, s, by_kernel, &stats.kernel_downloads) != CP_OK)
+    if (q.host_out_px && download(q.host_out_px, px, px_bytes, s, by_kernel, &wstats.kernel_downloads) != CP_OK)
         return fail(CP_ERR_HIP, "server: result download failed");
     if (q.host_out_mask && mask &&
-        download(q.host_out_mask, mask, mask_bytes, s, by_kernel, &stats.kernel_downloads) != CP_OK)
+        download(q.host_out_mask, mask, mask_bytes, s, by_kernel, &wstats.kernel_downloads) != CP_OK)
         return fail(CP_ERR_HIP, "server: mask download failed");
     if (hipEventRecord(slot.done, s) != hipSuccess) return fail(CP_ERR_HIP, "server: event record failed");
     return CP_OK;
 }
 
-void cp_server::retire(Slot& slot, int status) {
+// A slot's request is done (or failed): free its device memory in stream order.
+void cp_server::release_slot(Slot& slot, int status) {
     Request* r = slot.cur;
     hipStream_t s = slot.stream;
     if (r->own_frames) (void)hipFreeAsync(r->own_frames, s);
@@ -233,11 +250,11 @@ void cp_server::retire(Slot& slot, int status) {
     if (r->own_mask) (void)hipFreeAsync(r->own_mask, s);
     r->own_frames = r->own_px = r->own_mask = nullptr;
     slot.cur = nullptr;
-    finish(r, status);
+    release_request(r, status);
 }
 
-// Releases what is left of a request (not on a slot) and completes it.
-void cp_server::finish(Request* r, int status) {
+// Releases what is left of a request (not on a slot) and queues its completion.
+void cp_server::release_request(Request* r, int status) {
     if (r->own_frames) {  // an upload that failed or never reached a slot
         (void)hipFreeAsync(r->own_frames, up_stream);
         (void)hipStreamSynchronize(up_stream);
@@ -249,82 +266,99 @@ void cp_server::finish(Request* r, int status) {
     }
     used -= r->need;
     r->t_done = Clock::now();
-    r->status = status;
     if (status == CP_OK) {
-        stats.completed++;
-        stats.pixel_samples += r->pixel_samples;
+        wstats.completed++;
+        wstats.pixel_samples += r->pixel_samples;
     }
+    finished.emplace_back(r, status);
 }
 
 void cp_server::loop() {
     (void)hipSetDevice(ctx->device);
-    std::unique_lock<std::mutex> lock(mu);
+    std::vector<Request*> admitted;
     for (;;) {
-        bool busy = false, retired = false;
+        // ---- device progress, without the lock
+        bool busy = false;
         for (auto& sl : slots) {
             if (!sl.cur) continue;
-            hipError_t e = hipEventQuery(sl.done);
+            const hipError_t e = hipEventQuery(sl.done);
             if (e == hipSuccess) {
-                retire(sl, CP_OK);
-                retired = true;
+                release_slot(sl, CP_OK);
             } else if (e != hipErrorNotReady) {
                 sl.cur->error = hipGetErrorString(e);
-                retire(sl, CP_ERR_HIP);
-                retired = true;
+                release_slot(sl, CP_ERR_HIP);
             } else {
                 busy = true;
             }
         }
-        // uploads whose frames have arrived become ready for a slot
-        for (auto it = uploading.begin(); it != uploading.end();) {
+        for (auto it = uploading.begin(); it != uploading.end();) {  // arrived frames become ready
             Request* r = *it;
-            hipError_t e = hipEventQuery(r->uploaded);
+            const hipError_t e = hipEventQuery(r->uploaded);
             if (e == hipSuccess) {
                 ready.push_back(r);
                 it = uploading.erase(it);
             } else if (e != hipErrorNotReady) {
                 r->error = hipGetErrorString(e);
                 it = uploading.erase(it);
-                finish(r, CP_ERR_HIP);
-                retired = true;
+                release_request(r, CP_ERR_HIP);
             } else {
                 busy = true;
                 ++it;
             }
         }
-        // admission in arrival order while the budget allows
-        while (!queue.empty()) {
-            Request* r = queue.front();
-            if (budget && r->need > budget) {
+        // ---- shared state: completions, statistics, admission in arrival order
+        admitted.clear();
+        {
+            std::unique_lock<std::mutex> lock(mu);
+            for (auto& f : finished) f.first->status = f.second;
+            bool any_done = !finished.empty();
+            finished.clear();
+            while (!queue.empty()) {
+                Request* r = queue.front();
+                if (budget && r->need > budget) {
+                    queue.pop_front();
+                    r->status = CP_ERR_OUT_OF_HBM_BUDGET;
+                    r->error = "request needs more device memory than the server budget";
+                    r->t_done = Clock::now();
+                    wstats.rejected++;
+                    any_done = true;
+                    continue;
+                }
+                if (budget && used + r->need > budget) break;  // wait for memory to free up
+                // at most one upload in transit per slot: without a budget this bounds the memory
+                if (!r->req.job.stack && uploading.size() + admitted.size() >= slots.size()) break;
                 queue.pop_front();
-                r->status = CP_ERR_OUT_OF_HBM_BUDGET;
-                r->error = "request needs more device memory than the server budget";
-                r->t_done = Clock::now();
-                stats.rejected++;
-                retired = true;
-                continue;
+                used += r->need;
+                if (used > wstats.peak_bytes) wstats.peak_bytes = used;
+                admitted.push_back(r);
             }
-            if (budget && used + r->need > budget) break;  // wait for memory to free up
-            // at most one upload in transit per slot: without a budget this bounds the memory
-            if (!r->req.job.stack && uploading.size() >= slots.size()) break;
-            queue.pop_front();
-            used += r->need;
-            if (used > stats.peak_bytes) stats.peak_bytes = used;
+            const uint64_t submitted = stats.submitted;
+            stats = wstats;
+            stats.submitted = submitted;
+            if (any_done) cv_done.notify_all();
+            const bool idle = !busy && admitted.empty() && ready.empty() && uploading.empty();
+            if (idle) {
+                if (stop && queue.empty()) break;
+                if (queue.empty()) {
+                    cv_work.wait(lock, [&] { return stop || !queue.empty(); });
+                    continue;
+                }
+            }
+        }
+        // ---- staging and launches, without the lock
+        for (Request* r : admitted) {
             if (r->req.job.stack) {
                 ready.push_back(r);
                 continue;
             }
-            int rc = upload(r);
+            const int rc = upload(r);
             if (rc != CP_OK) {
                 r->error = cp_last_error();
-                finish(r, rc);
-                retired = true;
+                release_request(r, rc);
                 continue;
             }
             uploading.push_back(r);
-            busy = true;
         }
-        // ready requests onto free slots
         while (!ready.empty()) {
             Slot* free_slot = nullptr;
             for (auto& sl : slots)
@@ -333,25 +367,47 @@ void cp_server::loop() {
             Request* r = ready.front();
             ready.pop_front();
             free_slot->cur = r;
-            int rc = start(r, *free_slot);
+            const int rc = start(r, *free_slot);
             if (rc != CP_OK) {
                 r->error = cp_last_error();
                 (void)hipStreamSynchronize(free_slot->stream);
-                retire(*free_slot, rc);
-                retired = true;
+                release_slot(*free_slot, rc);
                 continue;
             }
-            busy = true;
             uint32_t inflight = 0;
             for (auto& sl : slots) inflight += sl.cur != nullptr;
-            if (inflight > stats.peak_inflight) stats.peak_inflight = inflight;
+            if (inflight > wstats.peak_inflight) wstats.peak_inflight = inflight;
         }
-        if (!ready.empty()) busy = true;
-        if (retired) cv_done.notify_all();
-        if (stop && queue.empty() && uploading.empty() && ready.empty() && !busy) break;
-        if (busy) cv_work.wait_for(lock, std::chrono::microseconds(20));
-        else cv_work.wait(lock, [&] { return stop || !queue.empty(); });
+        if (finished.empty()) {  // nothing to publish: give the device a moment (a submit wakes us)
+            std::unique_lock<std::mutex> lock(mu);
+            if (!stop) cv_work.wait_for(lock, std::chrono::microseconds(20));
+        }
     }
+}
+
+// Drains the queue and in-flight work, stops the worker and frees the streams.
+void cp_server::shutdown() {
+    if (stopped) return;
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        stop = true;
+        cv_work.notify_all();
+    }
+    if (worker.joinable()) worker.join();
+    for (auto& sl : slots) {
+        (void)hipStreamSynchronize(sl.stream);
+        (void)hipStreamDestroy(sl.stream);
+        (void)hipEventDestroy(sl.done);
+        sl.arena.release();
+    }
+    (void)hipStreamSynchronize(up_stream);
+    (void)hipStreamDestroy(up_stream);
+    stopped = true;
+}
+
+void cp::server_orphan(cp_server* v) {
+    v->shutdown();
+    v->ctx = nullptr;
 }
 
 extern "C" CP_API int cp_server_create(cp_ctx* ctx, uint64_t hbm_budget_bytes, uint32_t max_inflight,
@@ -380,14 +436,20 @@ extern "C" CP_API int cp_server_create(cp_ctx* ctx, uint64_t hbm_budget_bytes, u
     }
     cp_server* s = srv.release();
     s->worker = std::thread([s] { s->loop(); });
+    ctx->attach(s);
     *out = s;
     return CP_OK;
 }
 
 extern "C" CP_API int cp_server_submit(cp_server* srv, const cp_request* req, uint64_t* ticket) {
     if (!srv || !req || !ticket) return fail(CP_ERR_INVALID_ARG, "null argument");
+    if (!srv->ctx) return fail(CP_ERR_INVALID_ARG, "the server's context was destroyed");
     const cp_job& j = req->job;
     if (j.mode < CP_MODE_OUTLIER || j.mode > CP_MODE_DARKER) return fail(CP_ERR_INVALID_ARG, "bad mode");
+    // fade tables are read on the device: an empty or missing table is refused here, before
+    // anything is queued (cp_simple and cp_outlier refuse it the same way)
+    if (j.mode == CP_MODE_OUTLIER ? !fade_ok(j.outlier.fade) : !fade_ok(j.fade))
+        return fail(CP_ERR_INVALID_ARG, "fade table is empty");
     if (!j.stack) {
         if (!req->host_frames || req->n_frames == 0 || req->width == 0 || req->height == 0)
             return fail(CP_ERR_INVALID_ARG, "request without stack needs host frames");
@@ -461,20 +523,10 @@ extern "C" CP_API int cp_server_stats_get(cp_server* srv, cp_server_stats* out) 
 
 extern "C" CP_API int cp_server_destroy(cp_server* srv) {
     if (!srv) return CP_OK;
-    {
-        std::lock_guard<std::mutex> lock(srv->mu);
-        srv->stop = true;
-        srv->cv_work.notify_all();
+    if (srv->ctx) {  // else its context was destroyed first and already shut it down
+        srv->ctx->detach(srv);
+        srv->shutdown();
     }
-    if (srv->worker.joinable()) srv->worker.join();
-    for (auto& sl : srv->slots) {
-        (void)hipStreamSynchronize(sl.stream);
-        (void)hipStreamDestroy(sl.stream);
-        (void)hipEventDestroy(sl.done);
-        sl.arena.release();
-    }
-    (void)hipStreamSynchronize(srv->up_stream);
-    (void)hipStreamDestroy(srv->up_stream);
     delete srv;
     return CP_OK;
 }

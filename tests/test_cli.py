@@ -237,8 +237,9 @@ def test_cli_video_windows(tmp_path):
 @pytest.mark.gpu
 def test_cli_out_of_core_matches_resident(tmp_path):
     """--hbm-budget below the stack size: row bands stream from host memory and the image and
-    blend mask equal the resident run's."""
+    blend mask equal the oracle's and the resident run's."""
     st, pat = _frames(tmp_path, bps=2)
+    oimg, omask, _ = orc.outlier(st, orc.threshold(True, 0.05, 0.2, 2), 0, 4)
     outs = {}
     for name, extra in (("resident", ()), ("banded", ("--hbm-budget", "40K"))):
         out, blend = str(tmp_path / f"{name}.png"), str(tmp_path / f"{name}_b.png")
@@ -248,6 +249,8 @@ def test_cli_out_of_core_matches_resident(tmp_path):
         outs[name] = (read_png(out), read_png(blend))
         if name == "banded":
             assert "row bands" in r.stdout
+    np.testing.assert_array_equal(outs["banded"][0], oimg)
+    np.testing.assert_array_equal(outs["banded"][1], omask)
     np.testing.assert_array_equal(outs["banded"][0], outs["resident"][0])
     np.testing.assert_array_equal(outs["banded"][1], outs["resident"][1])
     for mode in ("darker", "lighter"):
@@ -257,6 +260,7 @@ def test_cli_out_of_core_matches_resident(tmp_path):
             r = _cli("--pattern", pat, "--output", out, "--mode", mode, *extra)
             assert r.returncode == 0, r.stderr
             dk[name] = read_png(out)
+        np.testing.assert_array_equal(dk["banded"], orc.simple(st, darker=mode == "darker"))
         np.testing.assert_array_equal(dk["banded"], dk["resident"])
     r = _cli("--pattern", pat, "--output", str(tmp_path / "s.png"), "--shake", "3/2", "--shake-anchors", "20/20",
              "--hbm-budget", "40K")

@@ -309,6 +309,66 @@ def test_partial_blends_with_dyadic_fades_are_exact_convex_steps():
     assert seen[1] > 3 and seen[2] > 10
 
 
+def test_relative_fade_reads_the_table_from_the_last_frame():
+    """src/chrono.rs:496-502: a relative fade takes Fade::get(N - t - 1) for frame t, so the
+    table runs backwards from the last frame.  Every outlier blends fully (min = max) and the
+    table gives frame t the dyadic value tvals[N - t - 1], so forward / backward must equal
+    the exact rational convex steps with those values, and a single outlier
+    round(bg + (s - bg) t) with t = tvals[N - t - 1].  An index of t or N - t would pick other
+    table entries and fail."""
+    from fractions import Fraction as Fr
+    rng = np.random.default_rng(41)
+    n = 18
+    x = _stack(rng, n, 6, 7, 3, 8, outlier_frac=0.2)
+    tvals = rng.choice([0.25, 0.5, 0.75], n)
+    assert len(set(tvals[:6])) > 1
+    fade = O.Fade(False, False, list(range(n)), [float(t) for t in tvals])
+    thr = O.threshold(True, 0.05, 0.05)
+    out = _outlier_sets(x, thr.min)
+    seen = 0
+    for mode in (4, 5):
+        img, mask, _ = O.outlier(x, thr, 0, mode, fade=fade)
+        for yy in range(x.shape[1]):
+            for xx in range(x.shape[2]):
+                ts = list(np.nonzero(out[:, yy, xx])[0])
+                if not ts or len(ts) == n or len(ts) > 8:
+                    continue
+                bg = x[[t for t in range(n) if t not in ts][0], yy, xx]
+                order = ts if mode == 4 else ts[::-1]
+                inv = Fr(1)
+                for t in order:
+                    inv *= 1 - Fr(float(tvals[n - t - 1]))
+                for ch in range(3):
+                    a = Fr(int(bg[ch]))
+                    for t in order:
+                        a = a + (int(x[t, yy, xx, ch]) - a) * Fr(float(tvals[n - t - 1]))
+                    assert img[yy, xx, ch] == _round_half_away(a.numerator, a.denominator), (mode, yy, xx, ch)
+                m = (1 - inv) * 255
+                assert np.all(mask[yy, xx] == _round_half_away(m.numerator, m.denominator)), (mode, yy, xx)
+                seen += 1
+    assert seen > 20
+
+
+@pytest.mark.parametrize("repeat", [False, True])
+def test_simple_relative_fade_by_hand(repeat):
+    """src/simple.rs:170-176 with Fade::get (src/options.rs:113-140): darker mode over seven
+    frames that each set a new extreme blends frame t with get(N - t - 1).  The table has three
+    entries (keys 0, 1, 2 -> 1/4, 1/2, 3/4), so indices 6..0 clamp to the last entry or wrap
+    modulo 3; the expected pixel is the sequence of rounded blends written out by hand."""
+    x = np.zeros((7, 1, 1, 1), np.uint8)
+    x[:, 0, 0, 0] = [240, 200, 160, 120, 80, 40, 0]
+    fade = O.Fade(repeat, False, [0, 1, 2], [0.25, 0.5, 0.75])
+    # index N - t - 1 = 6, 5, 4, 3, 2, 1, 0 for t = 0..6
+    table = {False: [0.75, 0.75, 0.75, 0.75, 0.75, 0.5, 0.25],   # clamp: indices >= 3 take entry 2
+             True: [0.25, 0.75, 0.5, 0.25, 0.75, 0.5, 0.25]}[repeat]  # repeat: index mod 3
+    o = 0
+    for t, v in enumerate(x[:, 0, 0, 0]):
+        assert fade.get(6 - t) == F32(table[t])
+        o = int(rround(F32(o) + (F32(int(v)) - F32(o)) * F32(table[t])))
+    img = O.simple(x, (1, 1, 1, 1), fade=fade, darker=True)
+    assert img[0, 0, 0] == o
+
+
 def test_average_mode_partial_blend_uses_frame_zero_fade():
     """src/chrono.rs:429-481: with two or more outliers the average mode blends the rounded
     outlier mean with the fade of frame 0 (the tuple's index is 0), with one outlier that
@@ -462,6 +522,39 @@ def test_shake_recovers_translations_and_matches_brute_force():
                 if best is None or tot < best:
                     best, bi = tot, (ox - s, oy - s)
         assert tuple(off[t]) == bi
+
+
+def test_shake_u8_sum_wraps_like_a_release_build_i32():
+    """src/shake.rs:350-379 sums squared differences of u8 samples in an i32 that wraps in the
+    reference's release build (Cargo.toml: overflow-checks = false).  One anchor of radius 52
+    (105 x 105 x 3 = 33,075 terms) against a frame of 255s gives 33,075 * 255^2 =
+    2,150,701,875 > 2^31 for the candidates that miss a column of zeros and 32,760 * 255^2 <
+    2^31 for those that contain it, so the exact arg-min (first minimum over (oy, ox) in
+    row-major order) and the arg-min of the sums read modulo 2^32 as i32 differ; the oracle
+    must return the wrapped one."""
+    r, s = 52, 1
+    h = w = 120
+    cx = cy = 60
+    x = np.zeros((2, h, w, 3), np.uint8)
+    x[1] = 255
+    x[1, :, cx - r - s, :] = 0  # only windows shifted by ox = -1 reach this column
+    exact, wrapped = {}, {}
+    for oy in range(2 * s + 1):
+        for ox in range(2 * s + 1):
+            win = x[0, cy - r:cy + r + 1, cx - r:cx + r + 1].astype(np.int64)
+            yy, xx = cy + oy - s, cx + ox - s
+            img = x[1, yy - r:yy + r + 1, xx - r:xx + r + 1].astype(np.int64)
+            tot = int(((win - img) ** 2).sum())
+            exact[(oy, ox)] = tot
+            wrapped[(oy, ox)] = (tot + (1 << 31)) % (1 << 32) - (1 << 31)
+    order = [(oy, ox) for oy in range(2 * s + 1) for ox in range(2 * s + 1)]
+    ex = min(order, key=lambda k: (exact[k], order.index(k)))
+    wr = min(order, key=lambda k: (wrapped[k], order.index(k)))
+    assert max(exact.values()) > (1 << 31) > min(exact.values())
+    assert ex != wr
+    off = O.shake(x, [(cx, cy)], r, s)
+    assert tuple(off[1]) == (wr[1] - s, wr[0] - s)
+    assert tuple(off[1]) != (ex[1] - s, ex[0] - s)
 
 
 def test_shake_anchor_out_of_range():

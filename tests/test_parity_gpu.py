@@ -123,11 +123,32 @@ def test_outlier_u8_full_tiles(ctx, n, c):
         assert_same(gpu, ref, f"n={n} c={c} abs={absolute} sel={sel}")
 
 
+def oracle_like(proc, x, slots=None, crop=None):
+    """The oracle's composite for an OutlierProcessor's options: (image, mask, stats, warnings)."""
+    bps = x.dtype.itemsize
+    t = proc.threshold
+    fade = O.Fade.none()
+    if not proc.fade.is_none:
+        raise ValueError("oracle_like covers composites without fade")
+    img, mask, warn, st = O.outlier(
+        x, O.threshold(t.absolute, t.min, t.max, bps), int(proc.background), int(proc.outlier),
+        weights=tuple(float(v) for v in proc.weights), fade=fade, seed=proc.seed, slots=slots,
+        crop_xy=None if crop is None else crop[0], out_w=None if crop is None else crop[1],
+        out_h=None if crop is None else crop[2], sample_pos=proc.sample_indices, stats=True)
+    return img, mask, st, warn
+
+
+def assert_host_matches_oracle(got, warnings, ref, msg):
+    """An out-of-core result (image, mask, stats) and its warning count against the oracle."""
+    img, mask, st = got
+    assert_same((img, mask, st, warnings), ref, msg)
+
+
 @pytest.mark.parametrize("bits,c", [(8, 3), (16, 3), (16, 4), (8, 1)])
 def test_outlier_host_bands_equal_resident(ctx, bits, c):
     """Out-of-core composites (row bands of host frames streamed through a band stack) equal
-    the resident composite: backgrounds incl. random, outlier modes, crops, a frame
-    selection, statistics and warnings, for bands of 1, 3 and all rows."""
+    the oracle and the resident composite: backgrounds incl. random, outlier modes, crops, a
+    frame selection, statistics and warnings, for bands of 1, 3 and all rows."""
     n, h, w = 40, 23, 37
     x = make_stack(1500 + bits + c, n, h, w, c, bits=bits)
     rng = np.random.default_rng(5)
@@ -139,9 +160,11 @@ def test_outlier_host_bands_equal_resident(ctx, bits, c):
         proc = cpa.OutlierProcessor(cpa.Threshold.abs(0.05, 0.2), bg, sel, seed=9)
         ref = proc.process(st, crop=crop, with_stats=True)
         ref_warn = proc.warnings
+        oref = oracle_like(proc, x, crop=crop)
         for band_rows in (1, 3, 64):
             band_bytes = (band_rows + 4) * row * n + 256  # crop halo of at most 4 rows + slack
             got = proc.process_host(ctx, x, crop=crop, with_stats=True, band_bytes=band_bytes)
+            assert_host_matches_oracle(got, proc.warnings, oref, f"bits={bits} c={c} bg={bg} sel={sel} band={band_rows}")
             for a, b, what in zip(got, ref, ("image", "mask", "stats")):
                 assert np.array_equal(a, b), (bits, c, bg, sel, band_rows, what)
             assert proc.warnings == ref_warn
@@ -149,7 +172,8 @@ def test_outlier_host_bands_equal_resident(ctx, bits, c):
     idx = [3, 4, 9, 10, 11, 20, 31, 32, 38]
     proc = cpa.OutlierProcessor(cpa.Threshold.abs(0.05, 0.2), 0, 0)
     ref = proc.process(st, image_indices=idx)
-    got = proc.process_host(ctx, x, image_indices=idx, band_bytes=2 * row * len(idx) + 256)
+    got = proc.process_host(ctx, x, image_indices=idx, band_bytes=2 * row * len(idx) + 256, with_stats=True)
+    assert_host_matches_oracle(got, proc.warnings, oracle_like(proc, x, slots=idx), "frame selection")
     assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
     with pytest.raises(cpa.ChronoError) as e:
         proc.process_host(ctx, x, crop=crop, band_bytes=row * n)
@@ -160,7 +184,7 @@ def test_outlier_host_bands_equal_resident(ctx, bits, c):
 def test_host_bands_random_configurations(ctx):
     """40 out-of-core composites from a fixed seed (band heights from 1 row to the whole
     image, crops, frame selections, outlier and darker/lighter modes, 1-4 channels) equal the
-    resident composite, statistics included."""
+    oracle and the resident composite, statistics included."""
     rng = np.random.default_rng(20261206)
     for case in range(40):
         bits = int(rng.choice([8, 16]))
@@ -192,6 +216,8 @@ def test_host_bands_random_configurations(ctx):
             ref = proc.process(st, image_indices=slots, crop=crop, with_stats=True)
             ref_warn = proc.warnings
             got = proc.process_host(ctx, x, image_indices=slots, crop=crop, with_stats=True, band_bytes=band_bytes)
+            assert_host_matches_oracle(got, proc.warnings, oracle_like(proc, x, slots=slots, crop=crop),
+                                       f"case {case}: bits={bits} c={c} n={n} {w}x{h} band={band_rows}")
             for a, b, what in zip(got, ref, ("image", "mask", "stats")):
                 assert np.array_equal(a, b), f"case {case}: {what} bits={bits} c={c} n={n} {w}x{h} band={band_rows}"
             assert proc.warnings == ref_warn, f"case {case}: warnings"
@@ -199,30 +225,38 @@ def test_host_bands_random_configurations(ctx):
             proc = cpa.SimpleProcessor(darker=bool(rng.random() < 0.5))
             ref = proc.process(st, image_indices=slots, crop=crop)
             got = proc.process_host(ctx, x, image_indices=slots, crop=crop, band_bytes=band_bytes)
+            oimg = O.simple(x, darker=proc.darker, slots=slots, crop_xy=None if crop is None else crop[0],
+                            out_w=None if crop is None else crop[1], out_h=None if crop is None else crop[2])
+            assert np.array_equal(got, oimg), f"case {case}: simple vs oracle bits={bits} c={c} n={n} {w}x{h}"
             assert np.array_equal(got, ref), f"case {case}: simple bits={bits} c={c} n={n} {w}x{h} band={band_rows}"
         st.close()
 
 
 @pytest.mark.parametrize("bits,c", [(8, 3), (16, 4)])
 def test_simple_host_bands_equal_resident(ctx, bits, c):
-    """Darker / lighter over host frames in row bands equal the resident composite, with crops,
-    a frame selection and a fade."""
+    """Darker / lighter over host frames in row bands equal the oracle and the resident
+    composite, with crops, a frame selection and a fade."""
     n, h, w = 30, 21, 33
     x = make_stack(1600 + bits + c, n, h, w, c, bits=bits)
     rng = np.random.default_rng(6)
     crop = (rng.integers(0, 4, (n, 2)).astype(np.int32), w - 3, h - 3)
     st = cpa.Stack.from_array(ctx, x)
     row = w * c * (bits // 8)
-    fade = cpa.Fade(False, True, [(0, 0.2), (12, 1.0), (25, 0.5)])
+    keys = [(0, 0.2), (12, 1.0), (25, 0.5)]
+    fade = cpa.Fade(False, True, keys)
+    ofade = O.Fade(False, True, [f for f, _ in keys], [v for _, v in keys])
     for darker in (True, False):
         proc = cpa.SimpleProcessor(fade=fade, darker=darker)
         ref = proc.process(st, crop=crop)
+        oimg = O.simple(x, fade=ofade, darker=darker, crop_xy=crop[0], out_w=crop[1], out_h=crop[2])
         for band_rows in (1, 5, 64):
             got = proc.process_host(ctx, x, crop=crop, band_bytes=(band_rows + 3) * row * n + 256)
+            assert np.array_equal(got, oimg), (bits, c, darker, band_rows, "oracle")
             assert np.array_equal(got, ref), (bits, c, darker, band_rows)
         idx = list(range(2, 28, 3))
         ref = proc.process(st, image_indices=idx)
         got = proc.process_host(ctx, x, image_indices=idx, band_bytes=4 * row * len(idx) + 256)
+        assert np.array_equal(got, O.simple(x, fade=ofade, darker=darker, slots=idx)), (bits, c, darker, "oracle frames")
         assert np.array_equal(got, ref), (bits, c, darker, "frames")
     st.close()
 
