@@ -142,7 +142,8 @@ ABI_SYMBOLS = [
     "cp_ctx_stream", "cp_stack_create", "cp_stack_wrap", "cp_stack_upload", "cp_stack_push",
     "cp_stack_commit", "cp_stack_download",
     "cp_stack_info_get", "cp_stack_destroy", "cp_threshold_new", "cp_threshold_blend_value",
-    "cp_fade_table", "cp_fade_get", "cp_outlier", "cp_outlier_host", "cp_simple", "cp_simple_host", "cp_shake", "cp_crop_create",
+    "cp_fade_table", "cp_fade_get", "cp_outlier", "cp_outlier_host", "cp_simple", "cp_simple_host", "cp_shake",
+    "cp_shake_noalloc", "cp_crop_create",
     "cp_video_windows", "cp_batch_run", "cp_batch_create", "cp_batch_launch", "cp_batch_destroy",
     "cp_autotune", "cp_tile_shapes_set", "cp_server_create", "cp_server_submit", "cp_server_poll",
     "cp_server_wait", "cp_server_stats_get", "cp_server_destroy",
@@ -200,6 +201,8 @@ def load_library() -> C.CDLL:
                                      C.c_uint64]),
         "cp_shake": (C.c_int, [C.c_void_p, C.c_void_p, P(C.c_uint32), C.c_uint32, P(C.c_int32),
                                C.c_uint32, C.c_uint32, C.c_uint32, P(C.c_int32)]),
+        "cp_shake_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, P(C.c_uint32), C.c_uint32, P(C.c_int32),
+                                       C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p, C.c_uint32]),
         "cp_crop_create": (C.c_int, [P(C.c_int32), C.c_uint32, C.c_uint32, C.c_uint32, P(C.c_int32),
                                      P(C.c_uint32), P(C.c_uint32), P(C.c_int32)]),
         "cp_video_windows": (C.c_int, [P(cp_frame_range), P(cp_frame_range), C.c_uint32, P(C.c_int32),

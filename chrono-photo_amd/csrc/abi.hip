@@ -226,6 +226,7 @@ CP_API int cp_ctx_create(int32_t device, uint64_t hbm_budget_bytes, cp_ctx** out
         return fail(CP_ERR_NO_DEVICE, "no HIP device available");
     if (device < 0 || device >= n) return fail(CP_ERR_INVALID_ARG, "device %d out of range", device);
     CP_HIP_TRY(hipSetDevice(device));
+    if (int rc = shake_scratch_init(device)) return rc;
     cp_ctx* c = new cp_ctx();
     memset(c->tuned, -1, sizeof(c->tuned));
     c->device = device;

@@ -124,4 +124,8 @@ int launch_simple(cp_ctx* ctx, const cp_stack* st, const SimpleK& k, hipStream_t
 void batch_orphan(cp_batch* b);
 void server_orphan(cp_server* v);
 
+// Looks up, once per device, the device address of cp_shake_noalloc's static scratch array
+// (shake.hip); called by cp_ctx_create with `device` current.
+int shake_scratch_init(int device);
+
 }  // namespace cp

@@ -1,12 +1,57 @@
-// shake.hip -- cp_shake: ShakeAnalyzer::analyze (src/shake.rs:190-313) on a resident stack.
+// shake.hip -- cp_shake: ShakeAnalyzer::analyze (src/shake.rs:190-313) on a resident stack,
+// and cp_shake_noalloc, the same search with its scratch in a static device array.
 #include "k_shake.h"
 #include "runtime.h"
 
 using namespace cp;
 
-extern "C" CP_API int cp_shake(cp_ctx* ctx, const cp_stack* st, const uint32_t* frames, uint32_t n,
-                               const int32_t* anchors_xy, uint32_t n_anchors, uint32_t anchor_radius,
-                               uint32_t search_radius, int32_t* offsets) {
+namespace {
+
+// Scratch of cp_shake_noalloc: the per-frame candidate sums ([n][(2s+1)^2] + 1 u64) and, for
+// host outputs, the offsets staged before their copy.  One array per device, for the life of
+// the process; its device address is looked up once per device when a context is created on
+// it (shake_scratch_init) and only that address is used on the host.
+constexpr uint64_t kShakeScratchBytes = 8ull << 20;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) unsigned long long scratch[kShakeScratchBytes / 8];
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Clears the sum table with a kernel (a memset node would misbehave if a caller captured the
+// call in a graph).
+__global__ void __launch_bounds__(256) zero_u64_kernel(unsigned long long* p, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) p[i] = 0;
+}
+
+// Offsets of a search without anchors or of a single frame: (0, 0) for position 0 and
+// (-s, -s) for the others (every sum is 0 and min_by_key keeps index 0).
+__global__ void __launch_bounds__(256) shake_trivial_kernel(int32_t* offsets, uint32_t n, int32_t s) {
+    for (uint32_t t = blockIdx.x * 256 + threadIdx.x; t < n; t += gridDim.x * 256) {
+        offsets[2 * t] = t == 0 ? 0 : -s;
+        offsets[2 * t + 1] = t == 0 ? 0 : -s;
+    }
+}
+
+}  // namespace
+
+int cp::shake_scratch_init(int device) {
+    if (device < 0 || device >= kMaxDevices) return fail(CP_ERR_INVALID_ARG, "device %d out of range", device);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return CP_OK;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return fail(CP_ERR_HIP, "scratch address lookup on device %d: %s", device, hipGetErrorString(e));
+    g_scratch_ptr[device] = p;
+    return CP_OK;
+}
+
+// The search of cp_shake.  scratch_dev == nullptr: the sum table comes from hipMalloc; else it
+// lives in scratch_dev (kShakeScratchBytes).  dev_out: offsets is a device array written
+// asynchronously on the context stream; else a host array and the call waits.
+static int shake_run(cp_ctx* ctx, const cp_stack* st, const uint32_t* frames, uint32_t n,
+                     const int32_t* anchors_xy, uint32_t n_anchors, uint32_t anchor_radius,
+                     uint32_t search_radius, int32_t* offsets, bool dev_out, void* scratch_dev) {
     if (!ctx || !st || !offsets || (!anchors_xy && n_anchors)) return fail(CP_ERR_INVALID_ARG, "null argument");
     if (st->ctx != ctx) return fail(CP_ERR_INVALID_ARG, "stack belongs to another context");
     if (n == 0) return fail(CP_ERR_INVALID_ARG, "shake analysis needs at least one frame");
@@ -30,18 +75,33 @@ extern "C" CP_API int cp_shake(cp_ctx* ctx, const cp_stack* st, const uint32_t* 
         if (slot >= st->count) return fail(CP_ERR_INVALID_ARG, "frame slot %u not loaded", slot);
         pos[t] = (uint64_t)slot * st->frame_elems();
     }
-    offsets[0] = 0;
-    offsets[1] = 0;
+    const int32_t ssize = (int32_t)(2 * s + 1);
+    const uint64_t noff = (uint64_t)ssize * ssize;
+    const uint64_t sums_bytes = (noff * n + 1) * 8;  // per-frame tables + the window constant
+    const uint64_t sums_al = (sums_bytes + 255) & ~uint64_t(255);
+    if (scratch_dev && sums_al + (dev_out ? 0 : 8ull * n) > kShakeScratchBytes)
+        return fail(CP_ERR_CAPACITY,
+                    "cp_shake_noalloc: %u frames x %llu candidate offsets need %llu bytes of scratch (at most %llu)",
+                    n, (unsigned long long)noff, (unsigned long long)(sums_al + (dev_out ? 0 : 8ull * n)),
+                    (unsigned long long)kShakeScratchBytes);
     if (n == 1 || n_anchors == 0) {
         // No anchors: every diff is 0 and min_by_key keeps index 0, i.e. offset (-s, -s).
+        if (dev_out) {
+            std::lock_guard<std::mutex> lock(ctx->mu);
+            (void)hipSetDevice(ctx->device);
+            hipLaunchKernelGGL(shake_trivial_kernel, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, offsets, n,
+                               (int32_t)s);
+            CP_HIP_TRY(hipGetLastError());
+            return CP_OK;
+        }
+        offsets[0] = 0;
+        offsets[1] = 0;
         for (uint32_t t = 1; t < n; ++t) { offsets[2 * t] = -(int32_t)s; offsets[2 * t + 1] = -(int32_t)s; }
         return CP_OK;
     }
     std::lock_guard<std::mutex> lock(ctx->mu);
     (void)hipSetDevice(ctx->device);
     hipStream_t hs = ctx->stream;
-    const int32_t ssize = (int32_t)(2 * s + 1);
-    const uint64_t noff = (uint64_t)ssize * ssize;
     ShakeK k{};
     k.stack = st->dev;
     k.pos_off = (const uint64_t*)ctx->arena.put(pos.data(), 8 * n, hs);
@@ -53,17 +113,28 @@ extern "C" CP_API int cp_shake(cp_ctx* ctx, const cp_stack* st, const uint32_t* 
     k.r = (int32_t)r;
     k.s = (int32_t)s;
     DeviceBuffer sums, offs;
-    const uint64_t sums_bytes = (noff * n + 1) * 8;  // per-frame tables + the window constant
-    int rc = ctx->charge(sums_bytes + n * 8);
-    if (rc) return rc;
-    if (hipMalloc(&sums.ptr, sums_bytes) != hipSuccess || hipMalloc(&offs.ptr, n * 8) != hipSuccess) {
-        if (sums.ptr) (void)hipFree(sums.ptr);
-        ctx->uncharge(sums_bytes + n * 8);
-        return fail(CP_ERR_HIP, "hipMalloc for shake sums failed");
+    int rc = CP_OK;
+    if (scratch_dev) {
+        sums.ptr = scratch_dev;
+        offs.ptr = dev_out ? (void*)offsets : (void*)((uint8_t*)scratch_dev + sums_al);
+    } else {
+        const uint64_t offs_bytes = dev_out ? 0 : 8ull * n;
+        if ((rc = ctx->charge(sums_bytes + offs_bytes))) return rc;
+        if (hipMalloc(&sums.ptr, sums_bytes) != hipSuccess ||
+            (!dev_out && hipMalloc(&offs.ptr, offs_bytes) != hipSuccess)) {
+            if (sums.ptr) (void)hipFree(sums.ptr);
+            ctx->uncharge(sums_bytes + offs_bytes);
+            return fail(CP_ERR_HIP, "hipMalloc for shake sums failed");
+        }
+        if (dev_out) offs.ptr = offsets;
     }
     k.sums = (unsigned long long*)sums.ptr;
     k.offsets = (int32_t*)offs.ptr;
-    hipError_t e = hipMemsetAsync(sums.ptr, 0, sums_bytes, hs);
+    // every sum starts at zero: the previous call's contents (a shared scratch) are cleared
+    const uint64_t n_sums = sums_bytes / 8;
+    hipLaunchKernelGGL(zero_u64_kernel, dim3((uint32_t)std::min<uint64_t>(1024, (n_sums + 255) / 256)), dim3(256), 0,
+                       hs, k.sums, n_sums);
+    hipError_t e = hipGetLastError();
     // Launch plan: strips of kShakeStrip offsets per lane, whole waves per row sub-chunk,
     // about 8 window rows per sub-chunk and all candidate rows per workgroup; where that
     // does not fit LDS, the larger of the two row counts shrinks.
@@ -123,13 +194,36 @@ extern "C" CP_API int cp_shake(cp_ctx* ctx, const cp_stack* st, const uint32_t* 
         if (wide) hipLaunchKernelGGL(shake_argmin_kernel<2>, dim3(n), dim3(256), 0, hs, k);
         else hipLaunchKernelGGL(shake_argmin_kernel<1>, dim3(n), dim3(256), 0, hs, k);
         e = hipGetLastError();
-        if (e == hipSuccess) e = hipMemcpyAsync(offsets, offs.ptr, n * 8, hipMemcpyDeviceToHost, hs);
-        if (e == hipSuccess) e = hipStreamSynchronize(hs);
+        if (e == hipSuccess && !dev_out) e = hipMemcpyAsync(offsets, offs.ptr, n * 8, hipMemcpyDeviceToHost, hs);
+        if (e == hipSuccess && !dev_out) e = hipStreamSynchronize(hs);
         if (e != hipSuccess && rc == CP_OK) rc = fail(CP_ERR_HIP, "shake kernels: %s", hipGetErrorString(e));
     }
-    (void)hipStreamSynchronize(hs);
-    (void)hipFree(sums.ptr);
-    (void)hipFree(offs.ptr);
-    ctx->uncharge(sums_bytes + n * 8);
+    if (!scratch_dev) {
+        (void)hipStreamSynchronize(hs);
+        (void)hipFree(sums.ptr);
+        if (!dev_out) (void)hipFree(offs.ptr);
+        ctx->uncharge(sums_bytes + (dev_out ? 0 : 8ull * n));
+    }
     return rc;
+}
+
+extern "C" CP_API int cp_shake(cp_ctx* ctx, const cp_stack* st, const uint32_t* frames, uint32_t n,
+                               const int32_t* anchors_xy, uint32_t n_anchors, uint32_t anchor_radius,
+                               uint32_t search_radius, int32_t* offsets) {
+    return shake_run(ctx, st, frames, n, anchors_xy, n_anchors, anchor_radius, search_radius, offsets, false,
+                     nullptr);
+}
+
+extern "C" CP_API int cp_shake_noalloc(cp_ctx* ctx, const cp_stack* st, const uint32_t* frames, uint32_t n,
+                                       const int32_t* anchors_xy, uint32_t n_anchors, uint32_t anchor_radius,
+                                       uint32_t search_radius, int32_t* offsets, uint32_t flags) {
+    if (!ctx) return fail(CP_ERR_INVALID_ARG, "null ctx");
+    void* p = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        if (ctx->device >= 0 && ctx->device < kMaxDevices) p = g_scratch_ptr[ctx->device];
+    }
+    if (!p) return fail(CP_ERR_HIP, "no scratch address for device %d", ctx->device);
+    return shake_run(ctx, st, frames, n, anchors_xy, n_anchors, anchor_radius, search_radius, offsets,
+                     (flags & CP_FLAG_DEVICE_OUT) != 0, p);
 }

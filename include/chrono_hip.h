@@ -195,6 +195,18 @@ CP_API int cp_simple_host(cp_ctx* ctx, const void* const* host_frames, uint32_t 
 CP_API int cp_shake(cp_ctx* ctx, const cp_stack* st, const uint32_t* frames, uint32_t n,
                     const int32_t* anchors_xy, uint32_t n_anchors, uint32_t anchor_radius,
                     uint32_t search_radius, int32_t* offsets);
+/* cp_shake without any device allocation (no counterpart in the reference): for callers that
+ * cannot allocate device memory -- code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc.  Same search and results as cp_shake; the per-frame
+ * candidate sums live in one static 8 MiB device array per GPU, so (8 n ((2s+1)^2) + 8,
+ * rounded up to 256, plus 8 n for host outputs) must fit 8 MiB, else CP_ERR_CAPACITY
+ * (s = 12: up to about 1,600 frames).  With CP_FLAG_DEVICE_OUT, offsets is a device array of
+ * 2 n int32 written asynchronously on the context stream; otherwise a host array and the call
+ * returns when done.  Every call on one GPU shares that array: calls that overlap on one GPU
+ * (other contexts or threads) race and must be serialised by the caller. */
+CP_API int cp_shake_noalloc(cp_ctx* ctx, const cp_stack* st, const uint32_t* frames, uint32_t n,
+                            const int32_t* anchors_xy, uint32_t n_anchors, uint32_t anchor_radius,
+                            uint32_t search_radius, int32_t* offsets, uint32_t flags);
 /* Crop::create (src/shake.rs:136-176): *has_crop = 0 where the reference returns None. */
 CP_API int cp_crop_create(const int32_t* offsets, uint32_t n, uint32_t width, uint32_t height,
                           int32_t* crop_xy, uint32_t* out_w, uint32_t* out_h, int32_t* has_crop);
