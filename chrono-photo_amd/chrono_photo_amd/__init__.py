@@ -896,20 +896,34 @@ def batch_run(ctx: Context, jobs):
     _check(ctx.lib.cp_batch_run(ctx.handle, arr, len(arr), 0))
 
 
+TILE_BUCKETS = (16, 32, 64, 128, 256, 512, 768, 1024, 1280)
+
+
+def _encode_shape(sh):
+    if sh is None:
+        return 0
+    g, f, x = sh[:3]
+    kind = sh[3] if len(sh) > 3 else 0
+    return (kind << 24) | (x << 16) | (g << 8) | f
+
+
 def tile_shapes_set(ctx: Context, bytes_per_sample: int, channels: int, shapes):
-    """cp_tile_shapes_set: pins the tile shape per bucket; shapes are (lanes_per_pixel,
-    frames_per_lane, pixels_per_lane) tuples as autotune returns them, or None for the
-    built-in default of that bucket."""
-    arr = (C.c_int32 * 6)(*[0 if sh is None else (sh[2] << 16) | (sh[0] << 8) | sh[1] for sh in shapes])
+    """cp_tile_shapes_set: pins the kernel choice per frame bucket (TILE_BUCKETS); each entry
+    is (lanes_per_pixel, frames_per_lane, pixels_per_lane[, kernel]) as autotune returns it
+    (kernel 1: the 8-bit packed-byte kernel, 0 or absent: the 16-bit-halves kernel), or None
+    for the bucket's default.  Missing trailing buckets take their defaults."""
+    shapes = list(shapes) + [None] * (len(TILE_BUCKETS) - len(shapes))
+    arr = (C.c_int32 * len(TILE_BUCKETS))(*[_encode_shape(sh) for sh in shapes])
     _check(ctx.lib.cp_tile_shapes_set(ctx.handle, bytes_per_sample, channels, arr))
 
 
 def autotune(ctx: Context, bytes_per_sample: int, channels: int):
-    """Runs the tile-shape autotuner; returns [(lanes_per_pixel, frames_per_lane,
-    pixels_per_lane)] per bucket (<=16, 32, 64, 128, 256, 512 frames)."""
-    out = (C.c_int32 * 6)()
+    """Runs the kernel autotuner; returns one (lanes_per_pixel, frames_per_lane,
+    pixels_per_lane, kernel) per bucket of TILE_BUCKETS, or None where no register-resident
+    kernel covers the bucket."""
+    out = (C.c_int32 * len(TILE_BUCKETS))()
     _check(ctx.lib.cp_autotune(ctx.handle, bytes_per_sample, channels, out))
-    return [((v >> 8) & 0xFF, v & 0xFF, v >> 16) for v in out]
+    return [None if v == 0 else ((v >> 8) & 0xFF, v & 0xFF, (v >> 16) & 0xFF, v >> 24) for v in out]
 
 
 # -------------------------------------------------------------------- serving

@@ -146,6 +146,7 @@ int prepare_outlier(const cp_stack* st, const cp_frames* frames, const cp_outlie
             out->sample_mask[t >> 5] |= 1u << (t & 31);
             prev = t;
         }
+        out->sample_pos.assign(p->sample_pos, p->sample_pos + p->n_sample);
     }
     if (p->fade.is_none == 0) {
         if (p->fade.len == 0 || !p->fade.values) return fail(CP_ERR_INVALID_ARG, "fade table is empty");
@@ -174,6 +175,24 @@ int prepare_outlier(const cp_stack* st, const cp_frames* frames, const cp_outlie
     return CP_OK;
 }
 
+int stage_outlier(ParamArena& arena, const OutlierPrep& prep, const std::vector<uint64_t>& pos, hipStream_t s,
+                  OutlierK* k) {
+    k->pos_off = (const uint64_t*)arena.put(pos.data(), 8 * pos.size(), s);
+    if (!k->pos_off) return fail(CP_ERR_HIP, "parameter staging failed");
+    if (!prep.sample_mask.empty()) {
+        k->sample_mask = (const uint32_t*)arena.put(prep.sample_mask.data(), 4 * prep.sample_mask.size(), s);
+        std::vector<uint64_t> so(prep.sample_pos.size());
+        for (size_t i = 0; i < so.size(); ++i) so[i] = pos[prep.sample_pos[i]];
+        k->samp_off = (const uint64_t*)arena.put(so.data(), 8 * so.size(), s);
+        if (!k->sample_mask || !k->samp_off) return fail(CP_ERR_HIP, "parameter staging failed");
+    }
+    if (!prep.fade_values.empty()) {
+        k->fade.values = (const float*)arena.put(prep.fade_values.data(), 4 * prep.fade_values.size(), s);
+        if (!k->fade.values) return fail(CP_ERR_HIP, "parameter staging failed");
+    }
+    return CP_OK;
+}
+
 }  // namespace cp
 
 int cp_ctx::ensure(cp::DeviceBuffer& b, size_t bytes) {
@@ -195,11 +214,11 @@ int cp_ctx::ensure(cp::DeviceBuffer& b, size_t bytes) {
     return CP_OK;
 }
 
-int cp_ctx::shape_for(uint32_t bps, uint32_t channels, uint32_t n) const {
+int cp_ctx::choice_for(uint32_t bps, uint32_t channels, uint32_t n) const {
     const int b = cp::bucket_for(n);
     if (b < 0 || bps < 1 || bps > 2 || channels < 1 || channels > 4) return -1;
     const int t = tuned[bps - 1][channels - 1][b];
-    return t >= 0 ? t : cp::kBucketCand[bps - 1][b][0];
+    return t >= 0 ? t : cp::kBucketCand[bps - 1][channels >= 3][b][0];
 }
 
 int cp_ctx::charge(uint64_t bytes) {
@@ -492,16 +511,7 @@ CP_API int cp_outlier(cp_ctx* ctx, const cp_stack* st, const cp_frames* frames,
     (void)hipSetDevice(ctx->device);
     hipStream_t s = ctx->stream;
     OutlierK k = prep.k;
-    k.pos_off = (const uint64_t*)ctx->arena.put(prep.pos_off.data(), 8 * prep.pos_off.size(), s);
-    if (!k.pos_off) return fail(CP_ERR_HIP, "parameter staging failed");
-    if (!prep.sample_mask.empty()) {
-        k.sample_mask = (const uint32_t*)ctx->arena.put(prep.sample_mask.data(), 4 * prep.sample_mask.size(), s);
-        if (!k.sample_mask) return fail(CP_ERR_HIP, "parameter staging failed");
-    }
-    if (!prep.fade_values.empty()) {
-        k.fade.values = (const float*)ctx->arena.put(prep.fade_values.data(), 4 * prep.fade_values.size(), s);
-        if (!k.fade.values) return fail(CP_ERR_HIP, "parameter staging failed");
-    }
+    if ((rc = stage_outlier(ctx->arena, prep, prep.pos_off, s, &k))) return rc;
     struct { uint32_t out_w, out_h; } g{k.out_w, k.out_h};
 
     const uint64_t npx = (uint64_t)g.out_w * g.out_h;

@@ -6,6 +6,7 @@
 #include <memory>
 
 #include "k_batch.h"
+#include "kernels_u8.h"
 #include "runtime.h"
 
 using namespace cp;
@@ -121,7 +122,8 @@ extern "C" CP_API int cp_batch_create(cp_ctx* ctx, const cp_job* jobs, uint32_t 
             OutlierPrep prep;
             int rc = prepare_outlier(j.stack, &j.frames, &j.outlier, &prep);
             if (rc) return rc;
-            int shape = ctx->shape_for(j.stack->bps, j.stack->channels, prep.k.n);
+            int shape = ctx->choice_for(j.stack->bps, j.stack->channels, prep.k.n);
+            if (shape < 0 || shape >= kU8Base) shape = fast_default_shape(j.stack->bps, prep.k.n);
             if (shape > kBatchShapeMax) shape = kBatchShapeMax;  // 8 x 32 covers up to 256 frames
             if (batch_fast_ok(j.stack, prep) && shape >= 0) {
                 Group* g = nullptr;
@@ -313,21 +315,30 @@ __global__ void autotune_fill_kernel(uint8_t* data, uint32_t bps, uint64_t n_ele
     }
 }
 
+// Public encoding of a kernel choice: (pixels per lane << 16) | (lanes per pixel << 8) |
+// frames per lane, with bit 24 set for the 8-bit packed-byte kernel.
+static int32_t encode_choice(int choice) {
+    if (choice >= kU8Base) {
+        const U8Shape& u = kU8Shapes[choice - kU8Base];
+        return (1 << 24) | (1 << 16) | (u.G << 8) | u.F;
+    }
+    return (kShapes[choice].X << 16) | (kShapes[choice].G << 8) | kShapes[choice].F;
+}
+
 extern "C" CP_API int cp_tile_shapes_set(cp_ctx* ctx, uint32_t bps, uint32_t channels, const int32_t* shapes) {
     if (!ctx || !shapes) return fail(CP_ERR_INVALID_ARG, "null argument");
-    if ((bps != 1 && bps != 2) || channels < 3 || channels > 4)
-        return fail(CP_ERR_INVALID_ARG, "tile shapes cover 8/16-bit stacks with 3 or 4 channels");
+    if ((bps != 1 && bps != 2) || channels < 1 || channels > 4)
+        return fail(CP_ERR_INVALID_ARG, "tile shapes cover 8/16-bit stacks with 1 to 4 channels");
     int8_t pick[kNumBuckets];
     for (int b = 0; b < kNumBuckets; ++b) {
         pick[b] = -1;
         if (shapes[b] == 0) continue;
-        for (int c = 0; c < 2; ++c) {
-            const int sh = kBucketCand[bps - 1][b][c];
-            if (sh >= 0 && shapes[b] == ((kShapes[sh].X << 16) | (kShapes[sh].G << 8) | kShapes[sh].F))
-                pick[b] = (int8_t)sh;
+        for (int c = 0; c < kMaxCand; ++c) {
+            const int ch = kBucketCand[bps - 1][channels >= 3][b][c];
+            if (ch >= 0 && shapes[b] == encode_choice(ch)) pick[b] = (int8_t)ch;
         }
         if (pick[b] < 0)
-            return fail(CP_ERR_INVALID_ARG, "shape 0x%06x is not a candidate for frame bucket %d", shapes[b], b);
+            return fail(CP_ERR_INVALID_ARG, "shape 0x%07x is not a candidate for frame bucket %d", shapes[b], b);
     }
     std::lock_guard<std::mutex> lock(ctx->mu);
     for (int b = 0; b < kNumBuckets; ++b) ctx->tuned[bps - 1][channels - 1][b] = pick[b];
@@ -336,9 +347,9 @@ extern "C" CP_API int cp_tile_shapes_set(cp_ctx* ctx, uint32_t bps, uint32_t cha
 
 extern "C" CP_API int cp_autotune(cp_ctx* ctx, uint32_t bps, uint32_t channels, int32_t* shapes) {
     if (!ctx) return fail(CP_ERR_INVALID_ARG, "null ctx");
-    if ((bps != 1 && bps != 2) || channels < 3 || channels > 4)
-        return fail(CP_ERR_INVALID_ARG, "autotune covers 8/16-bit stacks with 3 or 4 channels");
-    const uint32_t W = 2048, H = 24, NMAX = 512;
+    if ((bps != 1 && bps != 2) || channels < 1 || channels > 4)
+        return fail(CP_ERR_INVALID_ARG, "autotune covers 8/16-bit stacks with 1 to 4 channels");
+    const uint32_t W = 2048, H = 24, NMAX = kBucketMax[kNumBuckets - 1];
     cp_stack* st = nullptr;
     int rc = cp_stack_create(ctx, W, H, channels, bps, NMAX, &st);
     if (rc) return rc;
@@ -367,12 +378,16 @@ extern "C" CP_API int cp_autotune(cp_ctx* ctx, uint32_t bps, uint32_t channels, 
         ok = ok && hipEventCreate(&e0) == hipSuccess;
         ok = ok && hipEventCreate(&e1) == hipSuccess;
         ok = ok && hipMemcpyAsync(d_pos, pos.data(), pos_bytes, hipMemcpyHostToDevice, s) == hipSuccess;
+        auto run = [&](int choice, const OutlierK& k) {
+            if (choice >= kU8Base) launch_outlier_u8(channels, k, choice - kU8Base, s);
+            else launch_outlier_fast(bps, channels, k, choice, s);
+        };
         for (int b = 0; ok && b < kNumBuckets; ++b) {
-            int best = kBucketCand[bps - 1][b][0];
-            if (kBucketCand[bps - 1][b][1] >= 0) {
+            const int* cand = kBucketCand[bps - 1][channels >= 3][b];
+            int best = cand[0];
+            if (cand[1] >= 0) {
                 float best_ms = 1e30f;
-                for (int c = 0; ok && c < 2; ++c) {
-                    const int shape = kBucketCand[bps - 1][b][c];
+                for (int c = 0; ok && c < kMaxCand && cand[c] >= 0; ++c) {
                     OutlierK k{};
                     k.stack = st->dev;
                     k.pos_off = d_pos;
@@ -389,21 +404,21 @@ extern "C" CP_API int cp_autotune(cp_ctx* ctx, uint32_t bps, uint32_t channels, 
                     k.fade.is_none = 1;
                     k.out_px = d_out;
                     k.out_mask = (uint8_t*)d_out + fe * bps;
-                    launch_outlier_fast(bps, channels, k, shape, s);  // warm-up
+                    run(cand[c], k);  // warm-up
                     ok = hipEventRecord(e0, s) == hipSuccess;
-                    for (int r = 0; r < 3; ++r) launch_outlier_fast(bps, channels, k, shape, s);
+                    for (int r = 0; r < 3; ++r) run(cand[c], k);
                     ok = ok && hipEventRecord(e1, s) == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
                     float ms = 0.0f;
                     ok = ok && hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
                     if (ok && ms < best_ms) {
                         best_ms = ms;
-                        best = shape;
+                        best = cand[c];
                     }
                 }
             }
             if (!ok) break;
             ctx->tuned[bps - 1][channels - 1][b] = (int8_t)best;
-            if (shapes) shapes[b] = (kShapes[best].X << 16) | (kShapes[best].G << 8) | kShapes[best].F;
+            if (shapes) shapes[b] = best >= 0 ? encode_choice(best) : 0;
         }
         (void)hipStreamSynchronize(s);
         if (d_pos) (void)hipFree(d_pos);
@@ -412,7 +427,7 @@ extern "C" CP_API int cp_autotune(cp_ctx* ctx, uint32_t bps, uint32_t channels, 
         if (e1) (void)hipEventDestroy(e1);
         if (charged) ctx->uncharge(pos_bytes + out_bytes);
     }
-    cp_stack_destroy(st);  // takes no context lock of its own: outside the scope above for clarity
+    cp_stack_destroy(st);
     if (!charged) return rc;
     if (!ok) return fail(CP_ERR_HIP, "autotune failed: %s", hipGetErrorString(hipGetLastError()));
     return CP_OK;

@@ -44,13 +44,16 @@ typedef unsigned short us2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ us2 as_us2(uint32_t v) { return __builtin_bit_cast(us2, v); }
 __device__ __forceinline__ uint32_t as_u32(us2 v) { return __builtin_bit_cast(uint32_t, v); }
 
-// Sum / min / max over the G adjacent lanes of a pixel group (G in {1,2,4,8}); the DPP
-// moves fold into v_add/v_min/v_max_u32_dpp.  Every lane of the group receives the result.
+// Sum / min / max over the G adjacent lanes of a pixel group (G in {1,2,4,8,16}); the DPP
+// moves fold into v_add/v_min/v_max_u32_dpp.  Every lane of the group receives the result:
+// xor 1 and xor 2 within quads, then the half-row mirror (lane i <-> 7 - i) joins the two
+// quads of each half-row and the row mirror (lane i <-> 15 - i) the two half-rows.
 template <int G>
 __device__ __forceinline__ int32_t grp_sum(int32_t v) {
-    if constexpr (G >= 2) v += __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, true);   // xor 1
-    if constexpr (G >= 4) v += __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, true);   // xor 2
-    if constexpr (G >= 8) v += __builtin_amdgcn_update_dpp(0, v, 0x141, 0xF, 0xF, true);  // half-row mirror
+    if constexpr (G >= 2) v += __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, true);    // xor 1
+    if constexpr (G >= 4) v += __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, true);    // xor 2
+    if constexpr (G >= 8) v += __builtin_amdgcn_update_dpp(0, v, 0x141, 0xF, 0xF, true);   // half-row mirror
+    if constexpr (G >= 16) v += __builtin_amdgcn_update_dpp(0, v, 0x140, 0xF, 0xF, true);  // row mirror
     return v;
 }
 
@@ -59,6 +62,7 @@ __device__ __forceinline__ uint32_t grp_min_u32(uint32_t v) {
     if constexpr (G >= 2) v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)v, 0xB1, 0xF, 0xF, false));
     if constexpr (G >= 4) v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)v, 0x4E, 0xF, 0xF, false));
     if constexpr (G >= 8) v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)v, 0x141, 0xF, 0xF, false));
+    if constexpr (G >= 16) v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)v, 0x140, 0xF, 0xF, false));
     return v;
 }
 
@@ -67,6 +71,7 @@ __device__ __forceinline__ uint32_t grp_max_u32(uint32_t v) {
     if constexpr (G >= 2) v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false));
     if constexpr (G >= 4) v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, false));
     if constexpr (G >= 8) v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x141, 0xF, 0xF, false));
+    if constexpr (G >= 16) v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x140, 0xF, 0xF, false));
     return v;
 }
 
@@ -334,7 +339,7 @@ template <int C, int G, int F, int X = 1>
 struct TileShared {
     static constexpr int P = X * 256 / G;      // pixels per tile
     static constexpr int NS = G * P;           // lane summary slots (X per lane)
-    static constexpr int MW = F > 32 ? 2 : 1;  // mask words per lane and pixel
+    static constexpr int MW = (F + 31) / 32;   // mask words per lane and pixel
     static constexpr int OS = F + 2;           // offset-table stride of a lane block
     uint64_t off[G * OS];      // frame byte offsets at oidx(t); positions >= N repeat N-1
     // Two pixels per lane: with r the tile row's first sample address modulo 4 (shared by

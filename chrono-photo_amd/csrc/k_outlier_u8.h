@@ -1,0 +1,358 @@
+// k_outlier_u8.h -- register-resident outlier composite for 8-bit stacks of any channel count,
+// with up to 16 lanes per pixel (stacks of up to 16 x 80 frames) and an optional separate
+// block of the `--sample` frames.
+//
+// Layout.  A 256-lane workgroup covers P = 256 / G consecutive output pixels of one row; the
+// G adjacent lanes of a pixel each hold a contiguous block of F frames of its C channels as
+// packed bytes, y[c][i] byte j = channel c of frame g * F + 4 i + j, so a lane's F frames take
+// C * F / 4 VGPRs and the rank search (byte_select_multi) reads them as they lie.  Frames past
+// N are padded with 255, which never changes a rank below N.
+//
+// `--sample` (src/chrono.rs:151-163, 216-255): the medians and quartiles come from a subset
+// of the frame positions while the distances run over all of them.  With FS > 0 every lane
+// also loads an FS-frame block of the sample positions (offsets in samp_off) and the ranks
+// are searched there (padded with 255 past the subset); the classification still covers the
+// F-frame block of all positions.
+//
+// After the ranks each lane classifies its frames in the reference's f32 order
+// (src/chrono.rs:257-288), leaves its outlier bits, largest distance and sums in LDS, and the
+// per-pixel finish (finish_pixel, shared with the 16-bit kernel) selects the background and
+// blends (src/chrono.rs:290-493).
+#pragma once
+#include "k_outlier_fast.h"
+
+namespace cp {
+
+// v_perm_b32 selector: byte i of the result is byte s_i of {hi, lo} (0-3: lo, 4-7: hi).
+constexpr uint32_t psel(uint32_t s0, uint32_t s1, uint32_t s2, uint32_t s3) {
+    return s0 | (s1 << 8) | (s2 << 16) | (s3 << 24);
+}
+
+// Loads 4 frames' C bytes of one pixel (frame f at base + off[f]) and transposes them into
+// v[c] = the 4 frames' channel-c bytes.
+template <int C>
+__device__ __forceinline__ void load_quad_u8(const uint8_t* __restrict__ base, const uint64_t* off, uint32_t (&v)[C]) {
+    if constexpr (C == 4) {
+        uint32_t d[4];
+#pragma unroll
+        for (int f = 0; f < 4; ++f) d[f] = *reinterpret_cast<const uint32_t*>(base + off[f]);
+        const uint32_t a_lo = __builtin_amdgcn_perm(d[1], d[0], psel(0, 4, 1, 5));  // f0c0 f1c0 f0c1 f1c1
+        const uint32_t a_hi = __builtin_amdgcn_perm(d[1], d[0], psel(2, 6, 3, 7));  // f0c2 f1c2 f0c3 f1c3
+        const uint32_t b_lo = __builtin_amdgcn_perm(d[3], d[2], psel(0, 4, 1, 5));
+        const uint32_t b_hi = __builtin_amdgcn_perm(d[3], d[2], psel(2, 6, 3, 7));
+        v[0] = __builtin_amdgcn_perm(b_lo, a_lo, psel(0, 1, 4, 5));
+        v[1] = __builtin_amdgcn_perm(b_lo, a_lo, psel(2, 3, 6, 7));
+        v[2] = __builtin_amdgcn_perm(b_hi, a_hi, psel(0, 1, 4, 5));
+        v[3] = __builtin_amdgcn_perm(b_hi, a_hi, psel(2, 3, 6, 7));
+    } else if constexpr (C >= 2) {
+        uint32_t w[4];  // channels 0-1 of each frame
+#pragma unroll
+        for (int f = 0; f < 4; ++f) w[f] = *reinterpret_cast<const uint16_t*>(base + off[f]);
+        const uint32_t a = __builtin_amdgcn_perm(w[1], w[0], psel(0, 4, 1, 5));  // f0c0 f1c0 f0c1 f1c1
+        const uint32_t b = __builtin_amdgcn_perm(w[3], w[2], psel(0, 4, 1, 5));
+        v[0] = __builtin_amdgcn_perm(b, a, psel(0, 1, 4, 5));
+        v[1] = __builtin_amdgcn_perm(b, a, psel(2, 3, 6, 7));
+        if constexpr (C == 3) {
+            uint32_t e[4];
+#pragma unroll
+            for (int f = 0; f < 4; ++f) e[f] = base[off[f] + 2];
+            const uint32_t p = __builtin_amdgcn_perm(e[1], e[0], psel(0, 4, 0, 4));
+            const uint32_t q = __builtin_amdgcn_perm(e[3], e[2], psel(0, 4, 0, 4));
+            v[2] = __builtin_amdgcn_perm(q, p, psel(0, 1, 4, 5));
+        }
+    } else {
+        uint32_t e[4];
+#pragma unroll
+        for (int f = 0; f < 4; ++f) e[f] = base[off[f]];
+        const uint32_t p = __builtin_amdgcn_perm(e[1], e[0], psel(0, 4, 0, 4));
+        const uint32_t q = __builtin_amdgcn_perm(e[3], e[2], psel(0, 4, 0, 4));
+        v[0] = __builtin_amdgcn_perm(q, p, psel(0, 1, 4, 5));
+    }
+}
+
+// A lane's block of F frames (offsets off[0..F)) as packed bytes; frames from `valid` on are
+// set to 255 (valid is grid-uniform in the tile's block layout: N or the sample count).
+template <int C, int F>
+__device__ __forceinline__ void load_block_u8(const uint8_t* __restrict__ base, const uint64_t* off,
+                                              uint32_t (&y)[C][F / 4], uint32_t t0, uint32_t valid, bool pad) {
+#pragma unroll
+    for (int i = 0; i < F / 4; ++i) {
+        uint32_t v[C];
+        load_quad_u8<C>(base, off + 4 * i, v);
+#pragma unroll
+        for (int c = 0; c < C; ++c) y[c][i] = v[c];
+    }
+    if (pad) {
+#pragma unroll
+        for (int i = 0; i < F / 4; ++i) {
+            const int32_t rem = (int32_t)valid - (int32_t)(t0 + 4 * i);  // valid frames in this dword
+            const uint32_t m = rem >= 4 ? 0u : (rem <= 0 ? 0xFFFFFFFFu : (0xFFFFFFFFu << (8 * rem)));
+#pragma unroll
+            for (int c = 0; c < C; ++c) y[c][i] |= m;
+        }
+    }
+}
+
+// min{x > v} over the group's packed bytes (exists by the caller's count); the bytes are
+// widened to 16-bit halves, where values at or below v wrap to the top.
+template <int G, int ND>
+__device__ __forceinline__ uint32_t min_greater_u8(const uint32_t (&y)[ND], uint32_t v) {
+    const us2 V = {(unsigned short)(v + 1), (unsigned short)(v + 1)};
+    us2 m = {0xFFFF, 0xFFFF};
+#pragma unroll
+    for (int i = 0; i < ND; ++i) {
+        const uint32_t lo = y[i] & 0x00FF00FFu, hi = (y[i] >> 8) & 0x00FF00FFu;
+        m = __builtin_elementwise_min(m, as_us2(lo) - V);
+        m = __builtin_elementwise_min(m, as_us2(hi) - V);
+    }
+    const uint32_t mm = min((uint32_t)m.x, (uint32_t)m.y);
+    return grp_min_u32<G>(mm) + v + 1;
+}
+
+// Values at ranks (r0, r1), r1 in {r0, r0 + 1}, for the median and, for relative thresholds,
+// both quartiles of every channel, in one byte search whose C * R chains interleave.
+template <int G, int ND, int C, bool ABS>
+__device__ __forceinline__ void ranks_u8(const uint32_t (&y)[C][ND], const RankPlan& plan, int32_t ntot,
+                                         uint32_t (&m0)[C], uint32_t (&m1)[C], uint32_t (&a0)[C],
+                                         uint32_t (&a1)[C], uint32_t (&b0)[C], uint32_t (&b1)[C]) {
+    constexpr int R = ABS ? 1 : 3;
+    const uint32_t kk3[3] = {plan.m0, plan.a0, plan.b0};
+    const uint32_t(&kk)[R] = *reinterpret_cast<const uint32_t(*)[R]>(&kk3[0]);
+    uint32_t sel[C][R], cle[C][R];
+    byte_select_multi<G, ND, C, R, true>(y, kk, ntot, sel, cle);
+    auto nb = [&](int c, uint32_t v0, uint32_t cnt, uint32_t r0, uint32_t r1) -> uint32_t {
+        if (r1 == r0 || cnt >= r1 + 1) return v0;
+        return min_greater_u8<G, ND>(y[c], v0);
+    };
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        m0[c] = sel[c][0];
+        m1[c] = nb(c, m0[c], cle[c][0], plan.m0, plan.m1);
+        if constexpr (!ABS) {
+            a0[c] = sel[c][R > 1 ? 1 : 0];
+            a1[c] = nb(c, a0[c], cle[c][R > 1 ? 1 : 0], plan.a0, plan.a1);
+            b0[c] = sel[c][R > 2 ? 2 : 0];
+            b1[c] = nb(c, b0[c], cle[c][R > 2 ? 2 : 0], plan.b0, plan.b1);
+        } else {
+            a0[c] = a1[c] = b0[c] = b1[c] = 0;
+        }
+    }
+}
+
+// One tile: P = 256 / G pixels of row y from pixel bx * P.  sh.off holds the tile's frame
+// offsets (all positions) and soff the sample positions' offsets when FS > 0; every lane of
+// the workgroup calls this (it synchronises once).
+template <int C, int G, int F, int FS, bool ABS>
+__device__ __forceinline__ void outlier_u8_tile(const OutlierK& k, uint32_t bx, uint32_t y,
+                                                TileShared<C, G, F, 1>& sh, const uint64_t* soff) {
+    static_assert(F % 8 == 0 && F <= 96 && G <= 16 && FS % 4 == 0, "tile shape");
+    using TS = TileShared<C, G, F, 1>;
+    constexpr int P = TS::P;
+    constexpr int MW = TS::MW;
+    constexpr int ND = F / 4;
+    constexpr uint32_t NONE = 0xFFFFFFFFu;
+    const uint32_t N = k.n;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t g = lane % G;
+    const uint32_t t0 = g * F;
+    {
+        const uint32_t pl = lane / G;
+        const uint32_t x0 = bx * P + pl;
+        const uint64_t pix = ((uint64_t)y * k.stack_w + (x0 < k.out_w ? x0 : 0u)) * C;
+        const uint8_t* base = k.stack + pix;
+
+        uint32_t yv[C][ND];
+        load_block_u8<C, F>(base, sh.off + g * TS::OS, yv, t0, N, N < (uint32_t)(G * F));
+
+        // ---- medians / inverse IQR over the sample set
+        const RankPlan plan = rank_plan_core(k.n_sample, ABS);
+        uint32_t m0[C], m1[C], a0[C], a1[C], b0[C], b1[C];
+        if constexpr (FS > 0) {
+            constexpr int NDS = FS / 4;
+            uint32_t ys[C][NDS];
+            load_block_u8<C, FS>(base, soff + g * (FS + 2), ys, g * FS, k.n_sample, k.n_sample < (uint32_t)(G * FS));
+            ranks_u8<G, NDS, C, ABS>(ys, plan, G * FS, m0, m1, a0, a1, b0, b1);
+        } else {
+            ranks_u8<G, ND, C, ABS>(yv, plan, G * F, m0, m1, a0, a1, b0, b1);
+        }
+        float med[4], iqr_inv[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) med[c] = iqr_inv[c] = 0.0f;
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            float mc, ic;
+            stats_from_values(plan, m0[c], m1[c], a0[c], a1[c], b0[c], b1[c], ABS, &mc, &ic);
+            med[c] = k.w[c] == 0.0f ? 0.0f : mc;  // channels without weight keep zero statistics
+            iqr_inv[c] = k.w[c] == 0.0f ? 0.0f : ic;
+        }
+
+        // ---- classification over the lane's frames
+        float wi[4], sg[4];
+        bool unit = ABS, pos = true;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            sg[c] = signum_f(k.w[c]);
+            wi[c] = ABS ? k.w[c] : k.w[c] * iqr_inv[c];
+            if (c < C) {
+                unit = unit && k.w[c] == 1.0f;
+                pos = pos && !(sg[c] < 0.0f);
+            }
+        }
+        const float tsq = k.th.min * k.th.min;
+        const bool want_max = k.outlier == 2 || k.stats != nullptr;
+        const bool want_sums = k.background == 2 || k.outlier == 3;
+        const bool full_tile = N == (uint32_t)(G * F);
+        uint32_t m[MW];
+        float lmax = tsq;
+        uint32_t lmaxi = NONE;
+        uint32_t sum_all[4] = {0, 0, 0, 0}, sum_out[4] = {0, 0, 0, 0};
+#pragma unroll
+        for (int w = 0; w < MW; ++w) m[w] = 0;
+
+        // Frames run from last to first so that the mask shifts in bit j for frame j; the
+        // largest distance keeps the lowest frame among ties (>=).  lmax starts at the
+        // threshold, so `d >= lmax` also implies that the frame is an outlier.
+        auto classify = [&](auto unit_t, auto pos_t, auto max_t, auto sums_t, auto check_t) {
+            constexpr bool UNIT = decltype(unit_t)::value, POS = decltype(pos_t)::value;
+            constexpr bool MAXT = decltype(max_t)::value, SUMS = decltype(sums_t)::value;
+            constexpr bool CHECK = decltype(check_t)::value;
+#pragma unroll
+            for (int j = F - 1; j >= 0; --j) {
+                uint32_t xs[4];
+                float d = 0.0f;
+#pragma unroll
+                for (int c = 0; c < C; ++c) {
+                    xs[c] = (yv[c][j >> 2] >> (8 * (j & 3))) & 0xFFu;
+                    const float diff = med[c] - (float)xs[c];
+                    const float a = UNIT ? diff : wi[c] * diff;
+                    float term = a * a;
+                    if constexpr (!POS) term = sg[c] * term;
+                    d = c == 0 ? term : d + term;  // 0 + term0 == term0 up to the sign of zero
+                }
+                const bool o = d >= tsq;  // pads are masked out after the loop
+                m[j >> 5] = (m[j >> 5] << 1) | (o ? 1u : 0u);
+                bool in = true;
+                if constexpr (CHECK) in = t0 + (uint32_t)j < N;
+                if constexpr (MAXT) {
+                    if (in && d >= lmax) {
+                        lmax = d;
+                        lmaxi = t0 + (uint32_t)j;
+                    }
+                }
+                if constexpr (SUMS) {
+#pragma unroll
+                    for (int c = 0; c < C; ++c) sum_out[c] += (o && in) ? xs[c] : 0u;
+                }
+            }
+        };
+        using T_ = std::true_type;
+        using F_ = std::false_type;
+        // Bound check (lanes with few frames): with non-negative weights every frame's f32
+        // distance is at most the same f32 expression on the per-channel maximum deviation
+        // from the median (correct rounding is monotone), so below the threshold no frame of
+        // this lane is an outlier and the per-frame pass is skipped.
+        bool may_outlier = true;
+        if constexpr (F <= 32) {
+            if (pos && full_tile) {
+                float bound = 0.0f;
+#pragma unroll
+                for (int c = 0; c < C; ++c) {
+                    us2 mn = {0xFFFF, 0xFFFF}, mx = {0, 0};
+#pragma unroll
+                    for (int i = 0; i < ND; ++i) {
+                        const us2 lo = as_us2(yv[c][i] & 0x00FF00FFu), hi = as_us2((yv[c][i] >> 8) & 0x00FF00FFu);
+                        mn = __builtin_elementwise_min(mn, __builtin_elementwise_min(lo, hi));
+                        mx = __builtin_elementwise_max(mx, __builtin_elementwise_max(lo, hi));
+                    }
+                    const float lo = (float)min((uint32_t)mn.x, (uint32_t)mn.y);
+                    const float hi = (float)max((uint32_t)mx.x, (uint32_t)mx.y);
+                    const float dev = fmaxf(med[c] - lo, hi - med[c]);
+                    const float a = unit ? dev : wi[c] * dev;
+                    bound = c == 0 ? a * a : bound + a * a;
+                }
+                may_outlier = bound >= tsq;
+            }
+        }
+        if (may_outlier) {
+            if (!full_tile) classify(F_{}, F_{}, T_{}, T_{}, T_{});
+            else if (want_max) {
+                if (want_sums) classify(F_{}, F_{}, T_{}, T_{}, F_{});
+                else if (unit) classify(T_{}, T_{}, T_{}, F_{}, F_{});
+                else if (pos) classify(F_{}, T_{}, T_{}, F_{}, F_{});
+                else classify(F_{}, F_{}, T_{}, F_{}, F_{});
+            } else {
+                if (want_sums) classify(F_{}, F_{}, F_{}, T_{}, F_{});
+                else if (unit) classify(T_{}, T_{}, F_{}, F_{}, F_{});
+                else if (pos) classify(F_{}, T_{}, F_{}, F_{}, F_{});
+                else classify(F_{}, F_{}, F_{}, F_{}, F_{});
+            }
+        }
+        if (want_sums) {
+            // all-frame sums through v_sad_u8 against zero; pads (255) are taken back out
+            const uint32_t npad = t0 + F > N ? min(t0 + F - N, (uint32_t)F) : 0u;
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                uint32_t acc = 0;
+#pragma unroll
+                for (int i = 0; i < ND; ++i) acc = __builtin_amdgcn_sad_u8(yv[c][i], 0u, acc);
+                sum_all[c] = (uint32_t)grp_sum<G>((int32_t)(acc - npad * 255u));
+                sum_out[c] = (uint32_t)grp_sum<G>((int32_t)sum_out[c]);
+            }
+        }
+
+        // ---- leave the lane's summaries and the pixel's statistics in LDS
+        {
+            const int32_t nv = min((int32_t)N - (int32_t)t0, (int32_t)F);  // valid frames in this block
+            const uint32_t sl = TS::slot(g, pl);
+#pragma unroll
+            for (int w = 0; w < MW; ++w) {
+                const int32_t nw = nv - 32 * w;
+                const uint32_t vm = nw >= 32 ? 0xFFFFFFFFu : (nw <= 0 ? 0u : (1u << nw) - 1u);
+                sh.mask[w][sl] = m[w] & vm;
+            }
+            if (want_max) {
+                sh.lmax[sl] = lmax;
+                sh.lmaxi[sl] = lmaxi;
+            }
+        }
+        if (g == 0) {
+            const bool want_iqr = !ABS || k.stats != nullptr;
+            const bool want_s0 = k.background < 2;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                sh.med[c][pl] = med[c];
+                if (want_iqr) sh.iqr_inv[c][pl] = iqr_inv[c];
+            }
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                if (want_s0) sh.s0[c][pl] = yv[c][0] & 0xFFu;
+                if (want_sums) {
+                    sh.sum_all[c][pl] = sum_all[c];
+                    sh.sum_out[c][pl] = sum_out[c];
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (uint32_t p = lane; p < (uint32_t)P; p += 256) finish_pixel<1, C, G, F, 1, ABS>(k, bx, y, p, sh);
+}
+
+// VGPRs the frame blocks take, for the occupancy hint: C * (F + FS) / 4 plus working registers.
+constexpr int waves_hint_u8(int C, int F, int FS) {
+    const int w = 448 / (C * (F + FS) / 4 + 64);
+    return w < 1 ? 1 : (w > 8 ? 8 : w);
+}
+
+template <int C, int G, int F, int FS, bool ABS>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(waves_hint_u8(C, F, FS))))
+outlier_u8_kernel(OutlierK k, uint32_t tiles_x) {
+    __shared__ TileShared<C, G, F, 1> sh;
+    __shared__ uint64_t soff[FS > 0 ? G * (FS + 2) : 1];
+    const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
+    load_offsets<1, F>(sh.off, G * F, k.pos_off, k.n);
+    if constexpr (FS > 0) load_offsets<1, FS>(soff, G * FS, k.samp_off, k.n_sample);
+    __syncthreads();
+    outlier_u8_tile<C, G, F, FS, ABS>(k, tile % tiles_x, tile / tiles_x, sh, soff);
+}
+
+}  // namespace cp

@@ -2,6 +2,7 @@
 #include "k_outlier_generic.h"
 #include "k_simple.h"
 #include "kernels_fast.h"
+#include "kernels_u8.h"
 #include "runtime.h"
 
 namespace cp {
@@ -20,20 +21,31 @@ static int launch_outlier_generic_b(const cp_stack* st, const OutlierK& k, hipSt
 
 int launch_outlier(cp_ctx* ctx, const cp_stack* st, const OutlierK& k, uint64_t max_pos, uint32_t flags,
                    hipStream_t s) {
-    (void)ctx;
-    // The register-resident kernel needs the full frame set as the median sample and exact
-    // integer sums for the average modes (N * max sample < 2^24).
+    // The register-resident kernels need exact integer sums for the average modes
+    // (N * max sample < 2^24: always for 8-bit stacks).  8-bit composites take the packed-byte
+    // kernel up to 1280 frames (and, with a `--sample` subset, up to 128 sample frames) or the
+    // tuned choice between the two kernels; 16-bit ones the 16-bit kernel up to 512 frames
+    // without a sample subset; the one-lane-per-pixel generic kernel takes the rest.
     const bool sums = k.background == 2 || k.outlier == 3;
     const bool exact_sums = st->bps == 1 || k.n <= 256;
-    const bool fast_ok = !(flags & CP_FLAG_FORCE_GENERIC) && k.sample_mask == nullptr &&
-                         (!sums || exact_sums);
     bool done = false;
-    if (fast_ok) {
-        int shape = ctx->shape_for(st->bps, st->channels, k.n);
-        if (shape >= 0 && kShapes[shape].X == 2 &&
-            !pair_tile_fits(st->dev, st->alloc_bytes, st->bps, st->channels, k, max_pos))
-            shape = narrow_shape(shape);
-        if (shape >= 0) done = launch_outlier_fast(st->bps, st->channels, k, shape, s);
+    if (!(flags & CP_FLAG_FORCE_GENERIC) && (!sums || exact_sums)) {
+        if (k.sample_mask != nullptr) {
+            if (st->bps == 1 && k.n <= kU8MaxFrames && k.n_sample <= kU8MaxSample && k.samp_off) {
+                const int sh = u8_shape_for(k.n, true, k.n_sample);
+                if (sh >= 0) done = launch_outlier_u8(st->channels, k, sh, s);
+            }
+        } else {
+            int choice = ctx->choice_for(st->bps, st->channels, k.n);
+            if (choice >= kU8Base) {
+                done = launch_outlier_u8(st->channels, k, choice - kU8Base, s);
+            } else if (choice >= 0) {
+                if (kShapes[choice].X == 2 &&
+                    !pair_tile_fits(st->dev, st->alloc_bytes, st->bps, st->channels, k, max_pos))
+                    choice = narrow_shape(choice);
+                if (choice >= 0) done = launch_outlier_fast(st->bps, st->channels, k, choice, s);
+            }
+        }
     }
     if (!done) {
         if (st->bps == 1) launch_outlier_generic_b<1>(st, k, s);

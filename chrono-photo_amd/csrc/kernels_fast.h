@@ -21,17 +21,45 @@ constexpr int kNumShapes = 15;
 constexpr TileShape kShapes[kNumShapes] = {{1, 16, 1}, {1, 32, 1}, {2, 16, 1}, {2, 32, 1}, {4, 16, 1},
                                            {4, 32, 1}, {8, 16, 1}, {8, 32, 1}, {4, 64, 1}, {8, 64, 1},
                                            {1, 16, 2}, {1, 32, 2}, {2, 32, 2}, {4, 32, 2}, {8, 32, 2}};
-constexpr int kNumBuckets = 6;
-constexpr uint32_t kBucketMax[kNumBuckets] = {16, 32, 64, 128, 256, 512};
-// Candidates per bucket: [sample bytes - 1][bucket][2]
-constexpr int kBucketCand[2][kNumBuckets][2] = {{{0, -1}, {1, 2}, {3, 4}, {5, 6}, {7, 8}, {9, -1}},
-                                                {{10, 0}, {11, 1}, {12, 3}, {13, 5}, {14, 7}, {9, -1}}};
+// Frame-count buckets of the tuned kernel choice.  A choice is a shape of this kernel
+// (kShapes index) or, from kU8Base on, a shape of the 8-bit packed-byte kernel (kU8Shapes
+// index + kU8Base, kernels_u8.h); each bucket has up to kMaxCand candidates, the first being
+// the default, and cp_autotune keeps the fastest per (sample width, channels, bucket).
+constexpr int kNumBuckets = 9;
+constexpr uint32_t kBucketMax[kNumBuckets] = {16, 32, 64, 128, 256, 512, 768, 1024, 1280};
+constexpr int kU8Base = 64;
+constexpr int kMaxCand = 3;
+// [sample bytes - 1][channels >= 3][bucket][candidate]; -1: none
+constexpr int kBucketCand[2][2][kNumBuckets][kMaxCand] = {
+    // 8-bit, 1-2 channels: the packed-byte kernel only
+    {{{kU8Base + 0, -1, -1}, {kU8Base + 1, -1, -1}, {kU8Base + 2, -1, -1}, {kU8Base + 3, -1, -1},
+      {kU8Base + 4, -1, -1}, {kU8Base + 5, -1, -1}, {kU8Base + 6, -1, -1}, {kU8Base + 7, -1, -1},
+      {kU8Base + 8, -1, -1}},
+     // 8-bit, 3-4 channels
+     {{0, kU8Base + 0, -1}, {1, 2, kU8Base + 1}, {3, 4, kU8Base + 2}, {5, 6, kU8Base + 3}, {7, 8, kU8Base + 4},
+      {9, kU8Base + 5, -1}, {kU8Base + 6, -1, -1}, {kU8Base + 7, -1, -1}, {kU8Base + 8, -1, -1}}},
+    // 16-bit: 1-2 channels have no register-resident kernel; 3-4 channels up to 512 frames
+    {{{-1, -1, -1}, {-1, -1, -1}, {-1, -1, -1}, {-1, -1, -1}, {-1, -1, -1}, {-1, -1, -1}, {-1, -1, -1},
+      {-1, -1, -1}, {-1, -1, -1}},
+     {{10, 0, -1}, {11, 1, -1}, {12, 3, -1}, {13, 5, -1}, {14, 7, -1}, {9, -1, -1}, {-1, -1, -1},
+      {-1, -1, -1}, {-1, -1, -1}}}};
 // Shapes the batched kernels carry (F <= 32 keeps their register budget).
 constexpr int kBatchShapeMax = 7;
 
 inline int bucket_for(uint32_t n) {
     for (int b = 0; b < kNumBuckets; ++b)
         if (n <= kBucketMax[b]) return b;
+    return -1;
+}
+
+// The default choice of this kernel's shapes for n frames (batched kernels).
+inline int fast_default_shape(uint32_t bps, uint32_t n) {
+    const int b = bucket_for(n);
+    if (b < 0 || bps < 1 || bps > 2) return -1;
+    for (int c = 0; c < kMaxCand; ++c) {
+        const int ch = kBucketCand[bps - 1][1][b][c];
+        if (ch >= 0 && ch < kU8Base) return ch;
+    }
     return -1;
 }
 

@@ -34,6 +34,7 @@ struct OutlierK {
     const uint8_t* stack;
     const uint64_t* pos_off;     // n element offsets
     const uint32_t* sample_mask; // bit t set: position t is in the median sample (null: all)
+    const uint64_t* samp_off;    // element offsets of the n_sample sample positions (null: all)
     uint32_t n, n_sample;
     uint32_t stack_w;            // row pitch in pixels
     uint32_t out_w, out_h;

@@ -167,14 +167,9 @@ CP_API int cp_outlier_host(cp_ctx* ctx, const void* const* host_frames, uint32_t
     OutlierK k = prep.k;
     k.stack = bp.band.dev;
     k.stack_w = width;
-    k.pos_off = (const uint64_t*)ctx->arena.put(bp.pos.data(), 8 * npos, s);
-    if (!prep.sample_mask.empty())
-        k.sample_mask = (const uint32_t*)ctx->arena.put(prep.sample_mask.data(), 4 * prep.sample_mask.size(), s);
-    if (!prep.fade_values.empty())
-        k.fade.values = (const float*)ctx->arena.put(prep.fade_values.data(), 4 * prep.fade_values.size(), s);
-    if (!k.pos_off || (!prep.sample_mask.empty() && !k.sample_mask) || (!prep.fade_values.empty() && !k.fade.values)) {
+    if ((rc = stage_outlier(ctx->arena, prep, bp.pos, s, &k))) {
         free_bands(ctx, &bp, s);
-        return fail(CP_ERR_HIP, "parameter staging failed");
+        return rc;
     }
     k.warnings = (unsigned long long*)ctx->warn.ptr;
     if (hipMemsetAsync(ctx->warn.ptr, 0, 8, s) != hipSuccess) {

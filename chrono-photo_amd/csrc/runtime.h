@@ -67,9 +67,10 @@ struct cp_ctx {
     cp::ParamArena arena;
     cp::DeviceBuffer out_px, out_mask, out_stats, warn;  // host-output staging
     std::mutex mu;
-    // autotuned tile shape per (sample bytes - 1, channels - 1, frame bucket); -1 = default
-    int8_t tuned[2][4][8];
-    int shape_for(uint32_t bps, uint32_t channels, uint32_t n) const;
+    // autotuned kernel choice per (sample bytes - 1, channels - 1, frame bucket); -1 = default
+    int8_t tuned[2][4][16];
+    // the register-resident kernel choice for n frames (kernels_fast.h), or -1: none
+    int choice_for(uint32_t bps, uint32_t channels, uint32_t n) const;
     int ensure(cp::DeviceBuffer& b, size_t bytes);
     int charge(uint64_t bytes);
     void uncharge(uint64_t bytes) { used = used >= bytes ? used - bytes : 0; }
@@ -108,9 +109,15 @@ struct OutlierPrep {
     OutlierK k;
     std::vector<uint64_t> pos_off;
     std::vector<uint32_t> sample_mask;  // empty: every frame is in the median sample
+    std::vector<uint32_t> sample_pos;   // the sample positions, sorted (empty: all)
     std::vector<float> fade_values;     // empty: no fade
 };
 int prepare_outlier(const cp_stack* st, const cp_frames* frames, const cp_outlier_params* p, OutlierPrep* out);
+// Stages an outlier composite's per-call tables in `arena` on stream s -- the frame offsets
+// `pos` (the composite's, or a band stack's), the sample mask, the sample positions' offsets
+// taken from `pos`, the fade table -- and points k at the device copies.
+int stage_outlier(ParamArena& arena, const OutlierPrep& prep, const std::vector<uint64_t>& pos, hipStream_t s,
+                  OutlierK* k);
 FadeK fade_to_device(cp_ctx* ctx, const cp_fade* f, int* rc);
 
 // Kernel launchers (defined in the kernel translation units).

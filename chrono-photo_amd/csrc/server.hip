@@ -188,14 +188,7 @@ int cp_server::start(Request* r, Slot& slot) {
         OutlierPrep prep;
         if ((rc = prepare_outlier(st, &j.frames, &j.outlier, &prep))) return rc;
         OutlierK k = prep.k;
-        k.pos_off = (const uint64_t*)slot.arena.put(prep.pos_off.data(), 8 * prep.pos_off.size(), s);
-        if (!prep.sample_mask.empty())
-            k.sample_mask = (const uint32_t*)slot.arena.put(prep.sample_mask.data(), 4 * prep.sample_mask.size(), s);
-        if (!prep.fade_values.empty())
-            k.fade.values = (const float*)slot.arena.put(prep.fade_values.data(), 4 * prep.fade_values.size(), s);
-        if (!k.pos_off || (!prep.sample_mask.empty() && !k.sample_mask) ||
-            (!prep.fade_values.empty() && !k.fade.values))
-            return fail(CP_ERR_HIP, "server: parameter staging failed");
+        if ((rc = stage_outlier(slot.arena, prep, prep.pos_off, s, &k))) return rc;
         k.out_px = px;
         k.out_mask = mask;
         k.warnings = (unsigned long long*)j.out.warnings;

@@ -253,16 +253,19 @@ CP_API int cp_batch_destroy(cp_batch* batch);
 CP_API int cp_batch_run(cp_ctx* ctx, const cp_job* jobs, uint32_t n_jobs, uint32_t flags);
 #define CP_FLAG_NO_GRAPH 4u /* cp_batch_create: launch kernels directly, no graph capture */
 
-/* Autotuner: for each frame-count bucket of the register-resident outlier kernel
- * (<=16, 32, 64, 128, 256, 512 frames) time the candidate tile shapes on a synthetic stack
- * of this sample width and channel count and keep the fastest for the context.  shapes
- * (nullable, 6 entries) receives (pixels_per_lane << 16) | (lanes_per_pixel << 8) |
- * frames_per_lane per bucket. */
+/* Autotuner: for each frame-count bucket of the register-resident outlier kernels (<= 16, 32,
+ * 64, 128, 256, 512, 768, 1024, 1280 frames: CP_TILE_BUCKETS entries) time the candidate
+ * kernel choices on a synthetic stack of this sample width and channel count (1-4) and keep
+ * the fastest for the context.  A choice is a tile shape of the 16-bit-halves kernel or of
+ * the 8-bit packed-byte kernel; shapes (nullable, CP_TILE_BUCKETS entries) receives
+ * (kernel << 24) | (pixels_per_lane << 16) | (lanes_per_pixel << 8) | frames_per_lane per
+ * bucket (kernel 1: packed-byte), or 0 where no register-resident kernel covers the bucket. */
+#define CP_TILE_BUCKETS 9
 CP_API int cp_autotune(cp_ctx* ctx, uint32_t bytes_per_sample, uint32_t channels, int32_t* shapes);
-/* Sets the context's tile shape per bucket without timing anything (no reference
- * counterpart): shapes holds 6 entries in cp_autotune's encoding, for instance a result that
- * cp_autotune returned in an earlier process, or 0 for the built-in default of that bucket.
- * Each entry must be one of the bucket's two candidate shapes (CP_ERR_INVALID_ARG
+/* Sets the context's kernel choice per bucket without timing anything (no reference
+ * counterpart): shapes holds CP_TILE_BUCKETS entries in cp_autotune's encoding, for instance
+ * a result that cp_autotune returned in an earlier process, or 0 for the built-in default of
+ * that bucket.  Each entry must be one of the bucket's candidates (CP_ERR_INVALID_ARG
  * otherwise, and nothing is changed). */
 CP_API int cp_tile_shapes_set(cp_ctx* ctx, uint32_t bytes_per_sample, uint32_t channels, const int32_t* shapes);
 

@@ -103,8 +103,11 @@ def test_batch_against_oracle(ctx):
         assert np.array_equal(m, omask), spec
 
 
-CANDIDATES_U8 = [[(1, 16, 1)], [(1, 32, 1), (2, 16, 1)], [(2, 32, 1), (4, 16, 1)], [(4, 32, 1), (8, 16, 1)],
-                 [(8, 32, 1), (4, 64, 1)], [(8, 64, 1)]]
+# 8-bit 3-channel candidates per bucket (the packed-byte choices, kernel 1, make batched
+# groups fall back to the bucket's default shape of the batched kernel)
+CANDIDATES_U8 = [[(1, 16, 1), (1, 16, 1, 1)], [(1, 32, 1), (2, 16, 1), (1, 32, 1, 1)],
+                 [(2, 32, 1), (4, 16, 1), (2, 32, 1, 1)], [(4, 32, 1), (8, 16, 1), (4, 32, 1, 1)],
+                 [(8, 32, 1), (4, 64, 1), (8, 32, 1, 1)], [(8, 64, 1), (16, 32, 1, 1)]]
 
 
 def test_batch_with_every_candidate_shape_against_oracle(ctx):
@@ -112,7 +115,7 @@ def test_batch_with_every_candidate_shape_against_oracle(ctx):
     on stacks whose first frames are a run of outliers longer than a 16-frame block."""
     rng = np.random.default_rng(9)
     try:
-        for pick in (0, 1):
+        for pick in (0, 1, 2):
             cpa.tile_shapes_set(ctx, 1, 3, [c[min(pick, len(c) - 1)] for c in CANDIDATES_U8])
             jobs, keep = [], []
             for i, n in enumerate((20, 40, 64, 100, 128, 200)):
@@ -140,7 +143,7 @@ def test_batch_with_every_candidate_shape_against_oracle(ctx):
                 px.free()
                 mask.free()
     finally:
-        cpa.tile_shapes_set(ctx, 1, 3, [None] * 6)
+        cpa.tile_shapes_set(ctx, 1, 3, [])
 
 
 def test_batch_random_mixes(ctx):
@@ -198,10 +201,15 @@ def test_batch_random_mixes(ctx):
 @pytest.mark.parametrize("bps", [1, 2])
 def test_autotune_keeps_results(ctx, bps):
     shapes = cpa.autotune(ctx, bps, 3)
-    assert len(shapes) == 6
-    for (g, f, xp), nmax in zip(shapes, (16, 32, 64, 128, 256, 512)):
-        assert g * f >= nmax and g in (1, 2, 4, 8) and xp in ((1,) if bps == 1 else (1, 2))
-    for n in (12, 30, 60, 120, 250):
+    assert len(shapes) == len(cpa.TILE_BUCKETS)
+    for sh, nmax in zip(shapes, cpa.TILE_BUCKETS):
+        if bps == 2 and nmax > 512:
+            assert sh is None
+            continue
+        g, f, xp, kind = sh
+        assert g * f >= nmax and g in (1, 2, 4, 8, 16) and xp in ((1,) if bps == 1 else (1, 2))
+        assert kind in ((0, 1) if bps == 1 else (0,))
+    for n in (12, 30, 60, 120, 250) + ((700, 1200) if bps == 1 else ()):
         x, st = scene_stack(ctx, 7 + n, n, 70, 24, 3, bps)
         proc = cpa.OutlierProcessor(cpa.Threshold.abs(0.05, 0.2), 0, 4)
         img, mask = proc.process(st)

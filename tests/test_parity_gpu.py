@@ -317,13 +317,18 @@ def test_outlier_frame_counts(ctx, n):
         assert_same(gpu, ref, f"n={n} rel")
 
 
-# each frame bucket's two candidate tile shapes (lanes per pixel, frames per lane, pixels per lane)
+# each frame bucket's candidate kernel choices (lanes per pixel, frames per lane, pixels per
+# lane[, 1: the 8-bit packed-byte kernel]) for 3-4 channel stacks; buckets <= 16, 32, 64, 128,
+# 256, 512, 768, 1024, 1280 frames
 CANDIDATES = {
-    8: [[(1, 16, 1)], [(1, 32, 1), (2, 16, 1)], [(2, 32, 1), (4, 16, 1)], [(4, 32, 1), (8, 16, 1)],
-        [(8, 32, 1), (4, 64, 1)], [(8, 64, 1)]],
+    8: [[(1, 16, 1), (1, 16, 1, 1)], [(1, 32, 1), (2, 16, 1), (1, 32, 1, 1)],
+        [(2, 32, 1), (4, 16, 1), (2, 32, 1, 1)], [(4, 32, 1), (8, 16, 1), (4, 32, 1, 1)],
+        [(8, 32, 1), (4, 64, 1), (8, 32, 1, 1)], [(8, 64, 1), (16, 32, 1, 1)], [(16, 48, 1, 1)],
+        [(16, 64, 1, 1)], [(16, 80, 1, 1)]],
     16: [[(1, 16, 2), (1, 16, 1)], [(1, 32, 2), (1, 32, 1)], [(2, 32, 2), (2, 32, 1)], [(4, 32, 2), (4, 32, 1)],
-         [(8, 32, 2), (8, 32, 1)], [(8, 64, 1)]],
+         [(8, 32, 2), (8, 32, 1)], [(8, 64, 1)], [None], [None], [None]],
 }
+PICKS = (0, 1, 2)
 
 
 @pytest.mark.parametrize("bits,c", [(8, 3), (8, 4), (16, 3), (16, 4)])
@@ -333,10 +338,10 @@ def test_outlier_every_candidate_tile_shape(ctx, bits, c):
     is then the first frame after the run), constant and range-edge pixels."""
     hi = (1 << bits) - 1
     try:
-        for pick in (0, 1):
+        for pick in PICKS:
             shapes = [cands[min(pick, len(cands) - 1)] for cands in CANDIDATES[bits]]
             cpa.tile_shapes_set(ctx, bits // 8, c, shapes)
-            for n in (16, 32, 64, 128, 256) + ((512,) if bits == 8 else ()):
+            for n in (16, 32, 64, 128, 256) + ((512, 768, 1024, 1280) if bits == 8 else ()):
                 x = make_stack(2500 + n + c + bits, n, 4, 37, c, bits=bits, outlier_frac=0.05)
                 x[:, 1, 2:6, :] = hi // 2
                 x[: (n - 1) // 2, 2, 2:6, :] = hi // 40          # outliers over the first blocks
@@ -345,16 +350,16 @@ def test_outlier_every_candidate_tile_shape(ctx, bits, c):
                     gpu, ref = run_both(ctx, x, absolute, lo, thr_hi, bg, sel)
                     assert_same(gpu, ref, f"bits={bits} c={c} n={n} shape={shapes} abs={absolute}")
     finally:
-        cpa.tile_shapes_set(ctx, bits // 8, c, [None] * 6)
+        cpa.tile_shapes_set(ctx, bits // 8, c, [])
 
 
 @pytest.mark.parametrize("bits", [8, 16])
 def test_every_candidate_tile_shape_all_modes(ctx, bits):
-    """All background x selection modes, absolute and relative, under both candidate shapes
+    """All background x selection modes, absolute and relative, under every candidate choice
     of the 64- and 128-frame buckets."""
     hi = (1 << bits) - 1
     try:
-        for pick in (0, 1):
+        for pick in PICKS:
             shapes = [cands[min(pick, len(cands) - 1)] for cands in CANDIDATES[bits]]
             cpa.tile_shapes_set(ctx, bits // 8, 3, shapes)
             for n in (64, 128):
@@ -365,7 +370,7 @@ def test_every_candidate_tile_shape_all_modes(ctx, bits):
                         gpu, ref = run_both(ctx, x, absolute, lo, thr_hi, bg, sel)
                         assert_same(gpu, ref, f"bits={bits} n={n} pick={pick} bg={bg} sel={sel} abs={absolute}")
     finally:
-        cpa.tile_shapes_set(ctx, bits // 8, 3, [None] * 6)
+        cpa.tile_shapes_set(ctx, bits // 8, 3, [])
 
 
 def test_random_configurations(ctx):
@@ -402,18 +407,22 @@ def test_random_configurations(ctx):
             cr = O.crop_create(offs, w, h)
             if cr is not None and cr[1] > 0 and cr[2] > 0:  # +-2 on 4 rows can leave nothing
                 crop = cr
+        sample = None
+        if rng.random() < 0.25:  # `--sample`: medians over a sorted subset of the positions
+            k = int(rng.integers(3 if not absolute else 1, m + 1))
+            sample = sorted(rng.choice(m, k, replace=False).tolist())
         if c >= 3 and rng.random() < 0.5:
             cands = CANDIDATES[bits]
             cpa.tile_shapes_set(ctx, bits // 8, c, [cs[int(rng.integers(0, len(cs)))] for cs in cands])
         try:
             gpu, ref = run_both(ctx, x, absolute, lo, hi, bg, sel, weights=weights, fade=fade, slots=slots,
-                                crop=crop, seed=int(rng.integers(1 << 31)))
+                                crop=crop, sample=sample, seed=int(rng.integers(1 << 31)))
             assert_same(gpu, ref, f"case {case}: bits={bits} c={c} n={n} {w}x{h} abs={absolute} lo={lo} "
                                   f"hi={hi} bg={bg} sel={sel} w={weights} fade={fade} slots={slots} "
-                                  f"crop={crop is not None}")
+                                  f"crop={crop is not None} sample={sample}")
         finally:
             if c >= 3:
-                cpa.tile_shapes_set(ctx, bits // 8, c, [None] * 6)
+                cpa.tile_shapes_set(ctx, bits // 8, c, [])
 
 
 def test_random_two_pixel_tiles_with_crops(ctx):
@@ -482,11 +491,12 @@ def test_tile_shapes_set_rejects_non_candidates(ctx):
     assert e.value.code == cpa.CP_ERR_INVALID_ARG
 
 
-@pytest.mark.parametrize("bits,n", [(8, 513), (8, 700), (16, 300), (16, 520)])
-def test_outlier_tall_stacks_take_the_generic_kernel(ctx, bits, n):
-    """Stacks taller than the register tiles hold (over 512 frames), and 16-bit average modes
-    whose integer sums could leave f32's exact range (N * 65535 >= 2^24), run on the generic
-    kernel; every mode family stays exact there."""
+@pytest.mark.parametrize("bits,n", [(8, 513), (8, 700), (8, 1300), (16, 300), (16, 520)])
+def test_outlier_tall_stacks(ctx, bits, n):
+    """Tall stacks: 8-bit up to 1280 frames on the packed-byte kernel (16 lanes per pixel),
+    8-bit beyond it, 16-bit over 512 frames and 16-bit average modes whose integer sums could
+    leave f32's exact range (N * 65535 >= 2^24) on the generic kernel; every mode family stays
+    exact."""
     x = make_stack(2100 + n + bits, n, 4, 9, 3, bits=bits, outlier_frac=0.1)
     for absolute, lo, hi, bg, sel in ((True, 0.05, 0.2, 0, 4), (False, 2.0, 4.0, 2, 3),
                                       (True, 0.05, 0.2, 1, 2), (False, 1.0, 1.5, 3, 5)):
@@ -497,7 +507,7 @@ def test_outlier_tall_stacks_take_the_generic_kernel(ctx, bits, n):
 @pytest.mark.parametrize("bits,c", [(16, 3), (16, 1), (8, 2), (16, 4)])
 def test_outlier_generic_kernel_modes(ctx, bits, c):
     """The generic kernel forced on 16-bit and 1/2/4-channel stacks across the mode families
-    (it is also what 1- and 2-channel stacks always use)."""
+    (it is also what 16-bit 1- and 2-channel stacks always use)."""
     x = make_stack(2300 + bits + c, 23, 6, 11, c, bits=bits)
     for bg, sel in ((0, 0), (1, 1), (2, 3), (3, 4), (0, 5), (2, 2)):
         for absolute, lo, hi in ((True, 0.05, 0.2), (False, 2.0, 4.0)):

@@ -1,0 +1,100 @@
+"""The 8-bit packed-byte outlier kernel against the oracle: tall stacks up to 1280 frames
+(16 lanes per pixel), 1- and 2-channel stacks, `--sample` subsets ranked in their own block,
+every tile shape, partial tiles, weights, crops and frame selections.  Bit-exact."""
+import numpy as np
+import pytest
+
+import chrono_photo_amd as cpa
+import oracle_py as O
+from test_parity_gpu import MODES, assert_same, make_stack, run_both
+
+pytestmark = pytest.mark.gpu
+
+# the packed-byte choice of each bucket (lanes per pixel, frames per lane, pixels per lane, kernel)
+U8_CHOICE = [(1, 16, 1, 1), (1, 32, 1, 1), (2, 32, 1, 1), (4, 32, 1, 1), (8, 32, 1, 1), (16, 32, 1, 1),
+             (16, 48, 1, 1), (16, 64, 1, 1), (16, 80, 1, 1)]
+
+
+def _tall(seed, n, h, w, c, frac=0.1):
+    x = make_stack(seed, n, h, w, c, bits=8, outlier_frac=frac)
+    x[:, 1, 2:6, :] = 200                               # constant pixels
+    x[: (n - 1) // 2, 2, 2:6, :] = 10                   # outliers over the first half
+    x[:, 3, 2:6, :] = 255 - (np.arange(n) % 3)[:, None, None]  # at the top of the range
+    return x
+
+
+@pytest.mark.parametrize("n", [600, 777, 1024, 1280])
+@pytest.mark.parametrize("c", [3, 4, 1])
+def test_tall_stacks_all_modes(ctx, n, c):
+    """Every background x selection mode, absolute and relative thresholds, over stacks of
+    600-1280 frames (partial last lane blocks at 600 and 777)."""
+    x = _tall(3100 + n + c, n, 4, 19, c)
+    for bg, sel in MODES:
+        absolute = (bg + sel) % 2 == 0
+        lo, hi = (0.05, 0.2) if absolute else (2.0, 4.0)
+        gpu, ref = run_both(ctx, x, absolute, lo, hi, bg, sel)
+        assert_same(gpu, ref, f"n={n} c={c} bg={bg} sel={sel} abs={absolute}")
+
+
+@pytest.mark.parametrize("c", [1, 2])
+@pytest.mark.parametrize("n", [1, 3, 5, 16, 33, 64, 200, 512])
+def test_one_and_two_channel_stacks(ctx, c, n):
+    x = make_stack(3300 + n + c, n, 5, 23, c, bits=8)
+    for absolute, lo, hi, bg, sel in ((True, 0.05, 0.2, 0, 4), (False, 2.0, 4.0, 2, 3), (True, 0.1, 0.1, 1, 2),
+                                      (False, 1.0, 1.5, 3, 5), (True, 0.02, 0.3, 2, 0), (False, 0.5, 2.0, 0, 1)):
+        if not absolute and n < 3:
+            continue
+        gpu, ref = run_both(ctx, x, absolute, lo, hi, bg, sel, weights=(1.0, 0.5, 1, 1))
+        assert_same(gpu, ref, f"c={c} n={n} abs={absolute} bg={bg} sel={sel}")
+
+
+@pytest.mark.parametrize("n,s", [(40, 3), (40, 16), (300, 64), (300, 128), (1200, 64), (1280, 128), (1200, 1),
+                                 (300, 200)])
+@pytest.mark.parametrize("c", [3, 1, 4])
+def test_sample_subsets(ctx, n, s, c):
+    """`--sample`: medians and quartiles over a sorted subset of the positions, distances over
+    all of them; subsets of up to 128 frames take their own block of the packed-byte kernel,
+    larger ones the generic kernel."""
+    rng = np.random.default_rng(n * 7 + s + c)
+    x = _tall(3500 + n + s + c, n, 4, 21, c, frac=0.15)
+    sample = sorted(rng.choice(n, s, replace=False).tolist())
+    for bg, sel in ((0, 4), (2, 3), (1, 2), (3, 5), (0, 0), (2, 1)):
+        for absolute, lo, hi in ((True, 0.05, 0.2), (False, 2.0, 4.0)):
+            if not absolute and s < 3:
+                continue
+            gpu, ref = run_both(ctx, x, absolute, lo, hi, bg, sel, sample=sample)
+            assert_same(gpu, ref, f"n={n} s={s} c={c} bg={bg} sel={sel} abs={absolute}")
+
+
+@pytest.mark.parametrize("c", [3, 4])
+def test_every_packed_byte_shape(ctx, c):
+    """The packed-byte choice pinned in every bucket, at frame counts that fill each shape and
+    one frame short of it."""
+    try:
+        cpa.tile_shapes_set(ctx, 1, c, U8_CHOICE)
+        for nmax in cpa.TILE_BUCKETS:
+            for n in (nmax, nmax - 1):
+                x = _tall(3700 + n + c, n, 4, 33, c, frac=0.05)
+                for absolute, lo, hi, bg, sel in ((True, 0.05, 0.2, 0, 4), (False, 2.0, 4.0, 2, 3)):
+                    gpu, ref = run_both(ctx, x, absolute, lo, hi, bg, sel)
+                    assert_same(gpu, ref, f"c={c} n={n} abs={absolute}")
+    finally:
+        cpa.tile_shapes_set(ctx, 1, c, [])
+
+
+def test_tall_weights_crops_and_selections(ctx):
+    """Zero, negative and fractional weights, per-frame crops and a frame selection over a
+    tall stack, with fades on the selected frames."""
+    n, h, w = 900, 8, 41
+    x = _tall(3900, n, h, w, 3, frac=0.08)
+    rng = np.random.default_rng(39)
+    slots = sorted(rng.choice(n, 700, replace=False).tolist())
+    offs = rng.integers(-2, 3, (len(slots), 2)).tolist()
+    offs[0] = [0, 0]
+    crop = O.crop_create(offs, w, h)
+    fade = (False, True, [(slots[0], 0.25), (slots[0] + 300, 1.0), (slots[0] + 600, 0.5)])
+    for weights in ((1, 1, 1, 1), (0.5, 2.0, 0.0, 1.0), (1.0, -1.0, 1.0, 1.0)):
+        for absolute, lo, hi, bg, sel in ((True, 0.05, 0.2, 0, 4), (False, 2.0, 4.0, 2, 3), (True, 0.03, 0.1, 1, 5)):
+            gpu, ref = run_both(ctx, x, absolute, lo, hi, bg, sel, weights=weights, fade=fade, slots=slots,
+                                crop=crop)
+            assert_same(gpu, ref, f"weights={weights} abs={absolute} bg={bg} sel={sel}")
