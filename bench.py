@@ -177,11 +177,19 @@ def run_batching(cpa, H, ctx, kind="large"):
             "batched": "one graph replay: one launch per tile-shape group (cp_batch_create/cp_batch_launch)"}
 
 
-def run_serving(cpa, H, ctx, seed=4, n_jobs=24, window=60, n_video=600):
+def run_serving(cpa, H, ctx, seed=4, n_jobs=24, window=60, n_video=600, load=0.7, check_every=27):
     """A 600-frame RGB8 1080p chrono-video (60-frame windows, stride 1 -> 540 windows over
-    device-resident frames) interleaved with Poisson-arriving single-image jobs of 16-128
-    frames at 720p/1080p uploaded from pinned host memory; served (continuous batching,
-    HBM-budget admission) vs plain (one request at a time, image returned to the host)."""
+    device-resident frames) interleaved with single-image jobs of 16-128 frames at 720p/1080p
+    uploaded from pinned host memory.
+
+    * plain: one request at a time (cp_outlier, image copied back), each waited for;
+    * saturated: every request queued at once on cp_server (continuous batching, HBM-budget
+      admission): its throughput against plain is the batching gain;
+    * arrivals: the same requests submitted in wall time at seeded Poisson arrival times whose
+      offered pixel-sample rate is `load` x the saturated throughput; latency (submit to
+      result in host memory) is reported per class, windows and jobs;
+    * results: every `check_every`-th window and every job, in both served runs, must equal the
+      plain run's bytes."""
     import numpy as np
     rng = np.random.default_rng(seed)
     vsc = cpa.Scene(seed=seed, width=1920, height=1080, n_frames=n_video, n_disks=3, disk_r_min=10,
@@ -198,65 +206,101 @@ def run_serving(cpa, H, ctx, seed=4, n_jobs=24, window=60, n_video=600):
                        disk_r_max=30, blob_permille=100)
         st = _stack_from_scene(cpa, ctx, sc)
         pools[(w, h)] = H.PinnedArray((128, h, w, 3), np.uint8)
-        st_arr = st.download()
-        pools[(w, h)].array[...] = st_arr
+        pools[(w, h)].array[...] = st.download()
         st.close()
     jobs_spec = []
     t = 0.0
     for i in range(n_jobs):
-        t += rng.exponential(1.0 / n_jobs)  # arrival times over the run, in units of its length
+        t += rng.exponential(1.0 / n_jobs)  # position of the job among the windows
         n = int(rng.integers(16, 129))
         wh = (1280, 720) if rng.random() < 0.5 else (1920, 1080)
         jobs_spec.append((t, n, wh))
-    total = sum(len(ix) for ix in wins) * 1920 * 1080 + sum(n * wh[0] * wh[1] for _, n, wh in jobs_spec)
-    outs_w = [H.PinnedArray((1080, 1920, 3), np.uint8) for _ in range(4)]
+    # the request sequence: windows in order, each job after the window at its position
+    reqs, ji = [], 0
+    for k, ix in enumerate(wins):
+        reqs.append(("window", k))
+        while ji < len(jobs_spec) and jobs_spec[ji][0] * len(wins) <= k:
+            reqs.append(("job", ji))
+            ji += 1
+    samples = {("window", k): len(ix) * 1920 * 1080 for k, ix in enumerate(wins)}
+    samples.update({("job", i): n * wh[0] * wh[1] for i, (_, n, wh) in enumerate(jobs_spec)})
+    total = sum(samples[r] for r in reqs)
+    checked = [r for r in reqs if r[0] == "job" or r[1] % check_every == 0]
+    outs_w = [H.PinnedArray((1080, 1920, 3), np.uint8) for _ in range(4)]  # unchecked windows rotate
+    own_out = {r: H.PinnedArray((1080, 1920, 3) if r[0] == "window" else
+                                (jobs_spec[r[1]][2][1], jobs_spec[r[1]][2][0], 3), np.uint8) for r in checked}
+
     # ---- plain: one request at a time, image back on the host
-    lib = ctx.lib
+    plain_ref = {}
     t0 = time.perf_counter()
-    ji = 0
-    for k, ix in enumerate(wins):
-        img, _ = proc.process(video, image_indices=ix, with_mask=False)
-        while ji < len(jobs_spec) and jobs_spec[ji][0] * len(wins) <= k:
-            _, n, (w, h) = jobs_spec[ji]
+    for r in reqs:
+        if r[0] == "window":
+            img, _ = proc.process(video, image_indices=wins[r[1]], with_mask=False)
+        else:
+            _, n, (w, h) = jobs_spec[r[1]]
             st = cpa.Stack.from_array(ctx, pools[(w, h)].array[:n])
-            proc.process(st, with_mask=False)
+            img, _ = proc.process(st, with_mask=False)
             st.close()
-            ji += 1
+        if r in own_out:
+            plain_ref[r] = img
     plain_s = time.perf_counter() - t0
-    # ---- served
-    srv = cpa.Server(ctx, hbm_budget=16 << 30, max_inflight=8)
-    tickets, kinds = [], []
-    job_outs = [H.PinnedArray((h, w, 3), np.uint8) for _, _, (w, h) in jobs_spec]
-    t0 = time.perf_counter()
-    ji = 0
-    for k, ix in enumerate(wins):
-        job = cpa.Job(cpa.CP_MODE_OUTLIER, video, None, processor=proc, image_indices=ix)
-        tickets.append(srv.submit(job, host_out=outs_w[k % 4].array))
-        kinds.append("window")
-        while ji < len(jobs_spec) and jobs_spec[ji][0] * len(wins) <= k:
-            _, n, (w, h) = jobs_spec[ji]
-            job = cpa.Job(cpa.CP_MODE_OUTLIER, None, None, processor=proc, n_frames=n, bytes_per_sample=1)
-            tickets.append(srv.submit(job, host_frames=pools[(w, h)].array[:n],
-                                      host_out=job_outs[ji].array))
-            kinds.append("job")
-            ji += 1
-    lat = [srv.wait(tk) for tk in tickets]
-    served_s = time.perf_counter() - t0
-    st = srv.stats()
-    srv.close()
+
+    def submit(srv, r):
+        out = own_out[r].array if r in own_out else outs_w[r[1] % 4].array
+        if r[0] == "window":
+            job = cpa.Job(cpa.CP_MODE_OUTLIER, video, None, processor=proc, image_indices=wins[r[1]])
+            return srv.submit(job, host_out=out)
+        _, n, (w, h) = jobs_spec[r[1]]
+        job = cpa.Job(cpa.CP_MODE_OUTLIER, None, None, processor=proc, n_frames=n, bytes_per_sample=1)
+        return srv.submit(job, host_frames=pools[(w, h)].array[:n], host_out=out)
+
+    def served(arrivals=None):
+        for a in own_out.values():
+            a.array[...] = 0
+        srv = cpa.Server(ctx, hbm_budget=16 << 30, max_inflight=8)
+        tickets = []
+        t0 = time.perf_counter()
+        for i, r in enumerate(reqs):
+            if arrivals is not None:
+                while time.perf_counter() - t0 < arrivals[i]:
+                    pass
+            tickets.append(submit(srv, r))
+        lat = np.array([srv.wait(tk) for tk in tickets])
+        wall = time.perf_counter() - t0
+        st = srv.stats()
+        srv.close()
+        match = all(np.array_equal(own_out[r].array, plain_ref[r]) for r in checked)
+        return lat, wall, st, match
+
+    # ---- saturated: everything queued at once
+    lat_sat, sat_s, st_sat, match_sat = served()
+    # ---- Poisson arrivals at `load` x the saturated throughput
+    rate = load * total / sat_s  # offered pixel-samples per second
+    gaps = rng.exponential(1.0, len(reqs)) * np.array([samples[r] for r in reqs]) / rate
+    arrivals = np.cumsum(gaps) - gaps[0]
+    lat, arr_s, st_arr, match_arr = served(arrivals)
+    kinds = np.array([r[0] for r in reqs])
+
+    def pct(a, q):
+        return float(np.percentile(a, q)) if len(a) else None
+
     video.close()
-    lat = np.array(lat)
-    jl = np.array([l for l, k in zip(lat, kinds) if k == "job"])
-    return {"windows": len(wins), "window_frames": window, "jobs": len(jobs_spec),
-            "pixel_samples": int(total), "plain_value": total / plain_s, "served_value": total / served_s,
-            "speedup": plain_s / served_s, "p50_ms": float(np.percentile(lat, 50)),
-            "p99_ms": float(np.percentile(lat, 99)),
-            "job_p50_ms": float(np.percentile(jl, 50)) if len(jl) else None,
-            "job_p99_ms": float(np.percentile(jl, 99)) if len(jl) else None,
-            "peak_inflight": st["peak_inflight"], "peak_hbm_bytes": st["peak_bytes"],
+    for a in list(own_out.values()) + outs_w + list(pools.values()):
+        a.free()
+    return {"windows": len(wins), "window_frames": window, "jobs": len(jobs_spec), "pixel_samples": int(total),
+            "plain_value": total / plain_s, "served_value": total / sat_s, "speedup": plain_s / sat_s,
+            "saturated": {"p50_ms": pct(lat_sat, 50), "p99_ms": pct(lat_sat, 99),
+                          "peak_inflight": st_sat["peak_inflight"], "peak_hbm_bytes": st_sat["peak_bytes"],
+                          "kernel_downloads": st_sat["kernel_downloads"]},
+            "arrivals": {"process": "Poisson, seeded; mean gap = request pixel-samples / offered rate",
+                         "offered_load": load, "offered_value": rate, "achieved_value": total / arr_s,
+                         "window_p50_ms": pct(lat[kinds == "window"], 50),
+                         "window_p99_ms": pct(lat[kinds == "window"], 99),
+                         "job_p50_ms": pct(lat[kinds == "job"], 50), "job_p99_ms": pct(lat[kinds == "job"], 99),
+                         "peak_inflight": st_arr["peak_inflight"]},
+            "results_checked": len(checked) * 2, "results_match": bool(match_sat and match_arr),
             "plain": "one request at a time (cp_outlier, image copied to the host)",
-            "served": "cp_server: 8 stream slots, 16 GiB budget, continuous admission, uploads on their own stream",
-            "kernel_downloads": st["kernel_downloads"]}
+            "served": "cp_server: 8 stream slots, 16 GiB budget, continuous admission, uploads on their own stream"}
 
 
 def main():
@@ -424,9 +468,11 @@ def main():
               f"{extra['batching_small']['plain_ms']:.2f} -> {extra['batching_small']['batched_ms']:.2f} ms "
               f"({time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
         t0 = time.time()
-        extra["serving"] = run_serving(cpa, H, ctx)
-        print(f"[bench] serving: {extra['serving']['speedup']:.2f}x over plain, p50 "
-              f"{extra['serving']['p50_ms']:.2f} ms p99 {extra['serving']['p99_ms']:.2f} ms "
+        extra["serving"] = sv = run_serving(cpa, H, ctx)
+        print(f"[bench] serving: {sv['speedup']:.2f}x over plain saturated; Poisson arrivals at "
+              f"{sv['arrivals']['offered_load']:.0%} load: windows p50 {sv['arrivals']['window_p50_ms']:.2f} ms "
+              f"p99 {sv['arrivals']['window_p99_ms']:.2f} ms, jobs p50 {sv['arrivals']['job_p50_ms']:.2f} ms "
+              f"p99 {sv['arrivals']['job_p99_ms']:.2f} ms; results match: {sv['results_match']} "
               f"({time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
     traffic = measured_traffic(args.workload)
     res = {

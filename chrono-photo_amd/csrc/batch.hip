@@ -320,7 +320,7 @@ __global__ void autotune_fill_kernel(uint8_t* data, uint32_t bps, uint64_t n_ele
 static int32_t encode_choice(int choice) {
     if (choice >= kU8Base) {
         const U8Shape& u = kU8Shapes[choice - kU8Base];
-        return (1 << 24) | (1 << 16) | (u.G << 8) | u.F;
+        return (1 << 24) | (u.X << 16) | (u.G << 8) | u.F;
     }
     return (kShapes[choice].X << 16) | (kShapes[choice].G << 8) | kShapes[choice].F;
 }

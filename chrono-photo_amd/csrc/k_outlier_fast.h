@@ -44,16 +44,22 @@ typedef unsigned short us2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ us2 as_us2(uint32_t v) { return __builtin_bit_cast(us2, v); }
 __device__ __forceinline__ uint32_t as_u32(us2 v) { return __builtin_bit_cast(uint32_t, v); }
 
-// Sum / min / max over the G adjacent lanes of a pixel group (G in {1,2,4,8,16}); the DPP
+// Sum / min / max over the G adjacent lanes of a pixel group (G in {1,2,4,8,16,32}); the DPP
 // moves fold into v_add/v_min/v_max_u32_dpp.  Every lane of the group receives the result:
 // xor 1 and xor 2 within quads, then the half-row mirror (lane i <-> 7 - i) joins the two
-// quads of each half-row and the row mirror (lane i <-> 15 - i) the two half-rows.
+// quads of each half-row and the row mirror (lane i <-> 15 - i) the two half-rows; for 32
+// lanes v_permlane16_swap of the row results gives each lane of a row pair both rows
+// (its first result holds the even row of the pair, its second the odd row).
 template <int G>
 __device__ __forceinline__ int32_t grp_sum(int32_t v) {
     if constexpr (G >= 2) v += __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, true);    // xor 1
     if constexpr (G >= 4) v += __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, true);    // xor 2
     if constexpr (G >= 8) v += __builtin_amdgcn_update_dpp(0, v, 0x141, 0xF, 0xF, true);   // half-row mirror
     if constexpr (G >= 16) v += __builtin_amdgcn_update_dpp(0, v, 0x140, 0xF, 0xF, true);  // row mirror
+    if constexpr (G >= 32) {
+        const auto r = __builtin_amdgcn_permlane16_swap((uint32_t)v, (uint32_t)v, false, false);
+        v = (int32_t)(r[0] + r[1]);
+    }
     return v;
 }
 
@@ -63,6 +69,10 @@ __device__ __forceinline__ uint32_t grp_min_u32(uint32_t v) {
     if constexpr (G >= 4) v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)v, 0x4E, 0xF, 0xF, false));
     if constexpr (G >= 8) v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)v, 0x141, 0xF, 0xF, false));
     if constexpr (G >= 16) v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)v, 0x140, 0xF, 0xF, false));
+    if constexpr (G >= 32) {
+        const auto r = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+        v = min((uint32_t)r[0], (uint32_t)r[1]);
+    }
     return v;
 }
 
@@ -72,6 +82,10 @@ __device__ __forceinline__ uint32_t grp_max_u32(uint32_t v) {
     if constexpr (G >= 4) v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, false));
     if constexpr (G >= 8) v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x141, 0xF, 0xF, false));
     if constexpr (G >= 16) v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x140, 0xF, 0xF, false));
+    if constexpr (G >= 32) {
+        const auto r = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+        v = max((uint32_t)r[0], (uint32_t)r[1]);
+    }
     return v;
 }
 

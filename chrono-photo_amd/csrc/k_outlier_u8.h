@@ -1,12 +1,20 @@
 // k_outlier_u8.h -- register-resident outlier composite for 8-bit stacks of any channel count,
-// with up to 16 lanes per pixel (stacks of up to 16 x 80 frames) and an optional separate
+// with up to 32 lanes per pixel group (stacks of up to 1280 frames) and an optional separate
 // block of the `--sample` frames.
 //
-// Layout.  A 256-lane workgroup covers P = 256 / G consecutive output pixels of one row; the
-// G adjacent lanes of a pixel each hold a contiguous block of F frames of its C channels as
-// packed bytes, y[c][i] byte j = channel c of frame g * F + 4 i + j, so a lane's F frames take
-// C * F / 4 VGPRs and the rank search (byte_select_multi) reads them as they lie.  Frames past
-// N are padded with 255, which never changes a rank below N.
+// Layout.  A 256-lane workgroup covers P = X * 256 / G consecutive output pixels of one row;
+// the G adjacent lanes of a pixel group each hold a contiguous block of F frames of the
+// group's X pixels (C channels each) as packed bytes, y[e * C + c][i] byte j = channel c of
+// pixel e of frame g * F + 4 i + j, so a lane's F frames take X * C * F / 4 VGPRs and the
+// rank search (byte_select_multi) reads them as they lie.  Frames past N are padded with 255,
+// which never changes a rank below N.
+//
+// Loads.  The memory pipeline spends about the same time on a wave's load instruction
+// whatever its width, so a lane reads each frame's X * C bytes with one dword-aligned load
+// (8 or 16 bytes from the dword at or below them) and v_alignbyte drops the bytes before
+// them; with X = 4 a frame is 12 useful bytes per lane (RGB).  The loads may read up to 16
+// bytes past a tile row's last sample; the launcher checks that those bytes lie inside the
+// stack allocation (u8_tile_fits).
 //
 // `--sample` (src/chrono.rs:151-163, 216-255): the medians and quartiles come from a subset
 // of the frame positions while the distances run over all of them.  With FS > 0 every lane
@@ -28,59 +36,93 @@ constexpr uint32_t psel(uint32_t s0, uint32_t s1, uint32_t s2, uint32_t s3) {
     return s0 | (s1 << 8) | (s2 << 16) | (s3 << 24);
 }
 
-// Loads 4 frames' C bytes of one pixel (frame f at base + off[f]) and transposes them into
-// v[c] = the 4 frames' channel-c bytes.
-template <int C>
-__device__ __forceinline__ void load_quad_u8(const uint8_t* __restrict__ base, const uint64_t* off, uint32_t (&v)[C]) {
-    if constexpr (C == 4) {
-        uint32_t d[4];
-#pragma unroll
-        for (int f = 0; f < 4; ++f) d[f] = *reinterpret_cast<const uint32_t*>(base + off[f]);
-        const uint32_t a_lo = __builtin_amdgcn_perm(d[1], d[0], psel(0, 4, 1, 5));  // f0c0 f1c0 f0c1 f1c1
-        const uint32_t a_hi = __builtin_amdgcn_perm(d[1], d[0], psel(2, 6, 3, 7));  // f0c2 f1c2 f0c3 f1c3
-        const uint32_t b_lo = __builtin_amdgcn_perm(d[3], d[2], psel(0, 4, 1, 5));
-        const uint32_t b_hi = __builtin_amdgcn_perm(d[3], d[2], psel(2, 6, 3, 7));
-        v[0] = __builtin_amdgcn_perm(b_lo, a_lo, psel(0, 1, 4, 5));
-        v[1] = __builtin_amdgcn_perm(b_lo, a_lo, psel(2, 3, 6, 7));
-        v[2] = __builtin_amdgcn_perm(b_hi, a_hi, psel(0, 1, 4, 5));
-        v[3] = __builtin_amdgcn_perm(b_hi, a_hi, psel(2, 3, 6, 7));
-    } else if constexpr (C >= 2) {
-        uint32_t w[4];  // channels 0-1 of each frame
-#pragma unroll
-        for (int f = 0; f < 4; ++f) w[f] = *reinterpret_cast<const uint16_t*>(base + off[f]);
-        const uint32_t a = __builtin_amdgcn_perm(w[1], w[0], psel(0, 4, 1, 5));  // f0c0 f1c0 f0c1 f1c1
-        const uint32_t b = __builtin_amdgcn_perm(w[3], w[2], psel(0, 4, 1, 5));
-        v[0] = __builtin_amdgcn_perm(b, a, psel(0, 1, 4, 5));
-        v[1] = __builtin_amdgcn_perm(b, a, psel(2, 3, 6, 7));
-        if constexpr (C == 3) {
-            uint32_t e[4];
-#pragma unroll
-            for (int f = 0; f < 4; ++f) e[f] = base[off[f] + 2];
-            const uint32_t p = __builtin_amdgcn_perm(e[1], e[0], psel(0, 4, 0, 4));
-            const uint32_t q = __builtin_amdgcn_perm(e[3], e[2], psel(0, 4, 0, 4));
-            v[2] = __builtin_amdgcn_perm(q, p, psel(0, 1, 4, 5));
-        }
+// NW words of bytes from byte address a: word h holds bytes a + 4h .. a + 4h + 3.  They come
+// from the NQ dwords at or below a, realigned by a's offset in its dword: NQ = NW + 1 in
+// general, NQ = NW when a is dword-aligned (4-channel samples), NQ = NW = 1 when the bytes
+// needed lie in a's dword (one pixel of 1 or 2 channels).
+template <int NQ, int NW>
+__device__ __forceinline__ void load_realigned(const uint8_t* a, uint32_t (&w)[NW]) {
+    const uintptr_t ad = (uintptr_t)a;
+    const uint32_t sft = (uint32_t)ad & 3u;
+    const uintptr_t al = ad & ~(uintptr_t)3;
+    uint32_t q[NQ];
+    if constexpr (NQ == 1) {
+        q[0] = *reinterpret_cast<const uint32_t*>(al);
+    } else if constexpr (NQ == 2) {
+        const uint2 v = *reinterpret_cast<const uint2*>(al);
+        q[0] = v.x;
+        q[1] = v.y;
+    } else if constexpr (NQ == 3) {
+        typedef unsigned int u32x3 __attribute__((ext_vector_type(3)));
+        const u32x3 v = *reinterpret_cast<const u32x3*>(al);
+        q[0] = v.x;
+        q[1] = v.y;
+        q[2] = v.z;
     } else {
-        uint32_t e[4];
+        static_assert(NQ == 4, "at most 16 bytes per load");
+        const uint4 v = *reinterpret_cast<const uint4*>(al);
+        q[0] = v.x;
+        q[1] = v.y;
+        q[2] = v.z;
+        q[3] = v.w;
+    }
+    if constexpr (NQ == NW + 1) {
 #pragma unroll
-        for (int f = 0; f < 4; ++f) e[f] = base[off[f]];
-        const uint32_t p = __builtin_amdgcn_perm(e[1], e[0], psel(0, 4, 0, 4));
-        const uint32_t q = __builtin_amdgcn_perm(e[3], e[2], psel(0, 4, 0, 4));
-        v[0] = __builtin_amdgcn_perm(q, p, psel(0, 1, 4, 5));
+        for (int h = 0; h < NW; ++h) w[h] = __builtin_amdgcn_alignbyte(q[h + 1], q[h], sft);
+    } else if constexpr (NQ == 1) {
+        w[0] = __builtin_amdgcn_alignbyte(0u, q[0], sft);
+    } else {
+        static_assert(NQ == NW, "dword-aligned words");
+#pragma unroll
+        for (int h = 0; h < NW; ++h) w[h] = q[h];
+    }
+}
+
+// 4x4 byte transpose: out[k] byte f = byte k of in[f].
+__device__ __forceinline__ void transpose4(const uint32_t (&in)[4], uint32_t (&out)[4]) {
+    const uint32_t a_lo = __builtin_amdgcn_perm(in[1], in[0], psel(0, 4, 1, 5));  // f0b0 f1b0 f0b1 f1b1
+    const uint32_t a_hi = __builtin_amdgcn_perm(in[1], in[0], psel(2, 6, 3, 7));  // f0b2 f1b2 f0b3 f1b3
+    const uint32_t b_lo = __builtin_amdgcn_perm(in[3], in[2], psel(0, 4, 1, 5));
+    const uint32_t b_hi = __builtin_amdgcn_perm(in[3], in[2], psel(2, 6, 3, 7));
+    out[0] = __builtin_amdgcn_perm(b_lo, a_lo, psel(0, 1, 4, 5));
+    out[1] = __builtin_amdgcn_perm(b_lo, a_lo, psel(2, 3, 6, 7));
+    out[2] = __builtin_amdgcn_perm(b_hi, a_hi, psel(0, 1, 4, 5));
+    out[3] = __builtin_amdgcn_perm(b_hi, a_hi, psel(2, 3, 6, 7));
+}
+
+// Loads 4 frames (frame f at base + off[f]) of the lane's X pixels and transposes them into
+// v[e * C + c] = the 4 frames' bytes of pixel e, channel c.
+template <int C, int X>
+__device__ __forceinline__ void load_quad_u8(const uint8_t* __restrict__ base, const uint64_t* off,
+                                             uint32_t (&v)[X * C]) {
+    constexpr int NB = X * C;         // bytes per frame
+    constexpr int NW = (NB + 3) / 4;  // words per frame
+    constexpr int NQ = C == 4 ? NW : (X == 1 && C <= 2 ? 1 : NW + 1);  // dwords loaded
+    uint32_t w[4][NW];
+#pragma unroll
+    for (int f = 0; f < 4; ++f) load_realigned<NQ, NW>(base + off[f], w[f]);
+#pragma unroll
+    for (int h = 0; h < NW; ++h) {
+        const uint32_t in[4] = {w[0][h], w[1][h], w[2][h], w[3][h]};
+        uint32_t out[4];
+        transpose4(in, out);
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+            if (4 * h + b < NB) v[4 * h + b] = out[b];
     }
 }
 
 // A lane's block of F frames (offsets off[0..F)) as packed bytes; frames from `valid` on are
 // set to 255 (valid is grid-uniform in the tile's block layout: N or the sample count).
-template <int C, int F>
+template <int C, int X, int F>
 __device__ __forceinline__ void load_block_u8(const uint8_t* __restrict__ base, const uint64_t* off,
-                                              uint32_t (&y)[C][F / 4], uint32_t t0, uint32_t valid, bool pad) {
+                                              uint32_t (&y)[X * C][F / 4], uint32_t t0, uint32_t valid, bool pad) {
 #pragma unroll
     for (int i = 0; i < F / 4; ++i) {
-        uint32_t v[C];
-        load_quad_u8<C>(base, off + 4 * i, v);
+        uint32_t v[X * C];
+        load_quad_u8<C, X>(base, off + 4 * i, v);
 #pragma unroll
-        for (int c = 0; c < C; ++c) y[c][i] = v[c];
+        for (int c = 0; c < X * C; ++c) y[c][i] = v[c];
     }
     if (pad) {
 #pragma unroll
@@ -88,7 +130,7 @@ __device__ __forceinline__ void load_block_u8(const uint8_t* __restrict__ base, 
             const int32_t rem = (int32_t)valid - (int32_t)(t0 + 4 * i);  // valid frames in this dword
             const uint32_t m = rem >= 4 ? 0u : (rem <= 0 ? 0xFFFFFFFFu : (0xFFFFFFFFu << (8 * rem)));
 #pragma unroll
-            for (int c = 0; c < C; ++c) y[c][i] |= m;
+            for (int c = 0; c < X * C; ++c) y[c][i] |= m;
         }
     }
 }
@@ -139,17 +181,18 @@ __device__ __forceinline__ void ranks_u8(const uint32_t (&y)[C][ND], const RankP
     }
 }
 
-// One tile: P = 256 / G pixels of row y from pixel bx * P.  sh.off holds the tile's frame
+// One tile: P = X * 256 / G pixels of row y from pixel bx * P.  sh.off holds the tile's frame
 // offsets (all positions) and soff the sample positions' offsets when FS > 0; every lane of
 // the workgroup calls this (it synchronises once).
-template <int C, int G, int F, int FS, bool ABS>
+template <int C, int G, int F, int X, int FS, bool ABS>
 __device__ __forceinline__ void outlier_u8_tile(const OutlierK& k, uint32_t bx, uint32_t y,
-                                                TileShared<C, G, F, 1>& sh, const uint64_t* soff) {
-    static_assert(F % 8 == 0 && F <= 96 && G <= 16 && FS % 4 == 0, "tile shape");
-    using TS = TileShared<C, G, F, 1>;
+                                                TileShared<C, G, F, X>& sh, const uint64_t* soff) {
+    static_assert(F % 8 == 0 && F <= 96 && G <= 32 && FS % 4 == 0 && (X == 1 || X == 4), "tile shape");
+    using TS = TileShared<C, G, F, X>;
     constexpr int P = TS::P;
     constexpr int MW = TS::MW;
     constexpr int ND = F / 4;
+    constexpr int XC = X * C;
     constexpr uint32_t NONE = 0xFFFFFFFFu;
     const uint32_t N = k.n;
     const uint32_t lane = threadIdx.x;
@@ -157,42 +200,48 @@ __device__ __forceinline__ void outlier_u8_tile(const OutlierK& k, uint32_t bx, 
     const uint32_t t0 = g * F;
     {
         const uint32_t pl = lane / G;
-        const uint32_t x0 = bx * P + pl;
+        const uint32_t x0 = bx * P + pl * X;
         const uint64_t pix = ((uint64_t)y * k.stack_w + (x0 < k.out_w ? x0 : 0u)) * C;
         const uint8_t* base = k.stack + pix;
 
-        uint32_t yv[C][ND];
-        load_block_u8<C, F>(base, sh.off + g * TS::OS, yv, t0, N, N < (uint32_t)(G * F));
+        uint32_t yv[XC][ND];
+        load_block_u8<C, X, F>(base, sh.off + g * TS::OS, yv, t0, N, N < (uint32_t)(G * F));
 
-        // ---- medians / inverse IQR over the sample set
+        // ---- medians / inverse IQR over the sample set, every pixel and channel
         const RankPlan plan = rank_plan_core(k.n_sample, ABS);
-        uint32_t m0[C], m1[C], a0[C], a1[C], b0[C], b1[C];
+        uint32_t m0[XC], m1[XC], a0[XC], a1[XC], b0[XC], b1[XC];
         if constexpr (FS > 0) {
             constexpr int NDS = FS / 4;
-            uint32_t ys[C][NDS];
-            load_block_u8<C, FS>(base, soff + g * (FS + 2), ys, g * FS, k.n_sample, k.n_sample < (uint32_t)(G * FS));
-            ranks_u8<G, NDS, C, ABS>(ys, plan, G * FS, m0, m1, a0, a1, b0, b1);
+            uint32_t ys[XC][NDS];
+            load_block_u8<C, X, FS>(base, soff + g * (FS + 2), ys, g * FS, k.n_sample,
+                                    k.n_sample < (uint32_t)(G * FS));
+            ranks_u8<G, NDS, XC, ABS>(ys, plan, G * FS, m0, m1, a0, a1, b0, b1);
         } else {
-            ranks_u8<G, ND, C, ABS>(yv, plan, G * F, m0, m1, a0, a1, b0, b1);
+            ranks_u8<G, ND, XC, ABS>(yv, plan, G * F, m0, m1, a0, a1, b0, b1);
         }
-        float med[4], iqr_inv[4];
+        float med[X][4], iqr_inv[X][4];
 #pragma unroll
-        for (int c = 0; c < 4; ++c) med[c] = iqr_inv[c] = 0.0f;
+        for (int e = 0; e < X; ++e) {
 #pragma unroll
-        for (int c = 0; c < C; ++c) {
-            float mc, ic;
-            stats_from_values(plan, m0[c], m1[c], a0[c], a1[c], b0[c], b1[c], ABS, &mc, &ic);
-            med[c] = k.w[c] == 0.0f ? 0.0f : mc;  // channels without weight keep zero statistics
-            iqr_inv[c] = k.w[c] == 0.0f ? 0.0f : ic;
+            for (int c = 0; c < 4; ++c) med[e][c] = iqr_inv[e][c] = 0.0f;
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                float mc, ic;
+                const int q = e * C + c;
+                stats_from_values(plan, m0[q], m1[q], a0[q], a1[q], b0[q], b1[q], ABS, &mc, &ic);
+                med[e][c] = k.w[c] == 0.0f ? 0.0f : mc;  // channels without weight keep zero statistics
+                iqr_inv[e][c] = k.w[c] == 0.0f ? 0.0f : ic;
+            }
         }
 
         // ---- classification over the lane's frames
-        float wi[4], sg[4];
+        float wi[X][4], sg[4];
         bool unit = ABS, pos = true;
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
             sg[c] = signum_f(k.w[c]);
-            wi[c] = ABS ? k.w[c] : k.w[c] * iqr_inv[c];
+#pragma unroll
+            for (int e = 0; e < X; ++e) wi[e][c] = ABS ? k.w[c] : k.w[c] * iqr_inv[e][c];
             if (c < C) {
                 unit = unit && k.w[c] == 1.0f;
                 pos = pos && !(sg[c] < 0.0f);
@@ -202,17 +251,25 @@ __device__ __forceinline__ void outlier_u8_tile(const OutlierK& k, uint32_t bx, 
         const bool want_max = k.outlier == 2 || k.stats != nullptr;
         const bool want_sums = k.background == 2 || k.outlier == 3;
         const bool full_tile = N == (uint32_t)(G * F);
-        uint32_t m[MW];
-        float lmax = tsq;
-        uint32_t lmaxi = NONE;
-        uint32_t sum_all[4] = {0, 0, 0, 0}, sum_out[4] = {0, 0, 0, 0};
+        uint32_t m[X][MW];
+        float lmax[X];
+        uint32_t lmaxi[X];
+        uint32_t sum_all[X][4], sum_out[X][4];
 #pragma unroll
-        for (int w = 0; w < MW; ++w) m[w] = 0;
+        for (int e = 0; e < X; ++e) {
+#pragma unroll
+            for (int w = 0; w < MW; ++w) m[e][w] = 0;
+            lmax[e] = tsq;
+            lmaxi[e] = NONE;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) sum_all[e][c] = sum_out[e][c] = 0;
+        }
 
         // Frames run from last to first so that the mask shifts in bit j for frame j; the
         // largest distance keeps the lowest frame among ties (>=).  lmax starts at the
         // threshold, so `d >= lmax` also implies that the frame is an outlier.
-        auto classify = [&](auto unit_t, auto pos_t, auto max_t, auto sums_t, auto check_t) {
+        auto classify = [&](auto e_t, auto unit_t, auto pos_t, auto max_t, auto sums_t, auto check_t) {
+            constexpr int E = decltype(e_t)::value;
             constexpr bool UNIT = decltype(unit_t)::value, POS = decltype(pos_t)::value;
             constexpr bool MAXT = decltype(max_t)::value, SUMS = decltype(sums_t)::value;
             constexpr bool CHECK = decltype(check_t)::value;
@@ -222,137 +279,162 @@ __device__ __forceinline__ void outlier_u8_tile(const OutlierK& k, uint32_t bx, 
                 float d = 0.0f;
 #pragma unroll
                 for (int c = 0; c < C; ++c) {
-                    xs[c] = (yv[c][j >> 2] >> (8 * (j & 3))) & 0xFFu;
-                    const float diff = med[c] - (float)xs[c];
-                    const float a = UNIT ? diff : wi[c] * diff;
+                    xs[c] = (yv[E * C + c][j >> 2] >> (8 * (j & 3))) & 0xFFu;
+                    const float diff = med[E][c] - (float)xs[c];
+                    const float a = UNIT ? diff : wi[E][c] * diff;
                     float term = a * a;
                     if constexpr (!POS) term = sg[c] * term;
                     d = c == 0 ? term : d + term;  // 0 + term0 == term0 up to the sign of zero
                 }
                 const bool o = d >= tsq;  // pads are masked out after the loop
-                m[j >> 5] = (m[j >> 5] << 1) | (o ? 1u : 0u);
+                m[E][j >> 5] = (m[E][j >> 5] << 1) | (o ? 1u : 0u);
                 bool in = true;
                 if constexpr (CHECK) in = t0 + (uint32_t)j < N;
                 if constexpr (MAXT) {
-                    if (in && d >= lmax) {
-                        lmax = d;
-                        lmaxi = t0 + (uint32_t)j;
+                    if (in && d >= lmax[E]) {
+                        lmax[E] = d;
+                        lmaxi[E] = t0 + (uint32_t)j;
                     }
                 }
                 if constexpr (SUMS) {
 #pragma unroll
-                    for (int c = 0; c < C; ++c) sum_out[c] += (o && in) ? xs[c] : 0u;
+                    for (int c = 0; c < C; ++c) sum_out[E][c] += (o && in) ? xs[c] : 0u;
                 }
             }
         };
         using T_ = std::true_type;
         using F_ = std::false_type;
-        // Bound check (lanes with few frames): with non-negative weights every frame's f32
-        // distance is at most the same f32 expression on the per-channel maximum deviation
-        // from the median (correct rounding is monotone), so below the threshold no frame of
-        // this lane is an outlier and the per-frame pass is skipped.
-        bool may_outlier = true;
-        if constexpr (F <= 32) {
-            if (pos && full_tile) {
-                float bound = 0.0f;
+        StaticFor<0, X>::run([&](auto e_t) {
+            constexpr int E = decltype(e_t)::value;
+            // Bound check (lanes with few frames): with non-negative weights every frame's f32
+            // distance is at most the same f32 expression on the per-channel maximum deviation
+            // from the median (correct rounding is monotone), so below the threshold no frame
+            // of this lane is an outlier and the per-frame pass is skipped.
+            bool may_outlier = true;
+            if constexpr (F <= 32) {
+                if (pos && full_tile) {
+                    float bound = 0.0f;
 #pragma unroll
-                for (int c = 0; c < C; ++c) {
-                    us2 mn = {0xFFFF, 0xFFFF}, mx = {0, 0};
+                    for (int c = 0; c < C; ++c) {
+                        us2 mn = {0xFFFF, 0xFFFF}, mx = {0, 0};
 #pragma unroll
-                    for (int i = 0; i < ND; ++i) {
-                        const us2 lo = as_us2(yv[c][i] & 0x00FF00FFu), hi = as_us2((yv[c][i] >> 8) & 0x00FF00FFu);
-                        mn = __builtin_elementwise_min(mn, __builtin_elementwise_min(lo, hi));
-                        mx = __builtin_elementwise_max(mx, __builtin_elementwise_max(lo, hi));
+                        for (int i = 0; i < ND; ++i) {
+                            const uint32_t v = yv[E * C + c][i];
+                            const us2 lo = as_us2(v & 0x00FF00FFu), hi = as_us2((v >> 8) & 0x00FF00FFu);
+                            mn = __builtin_elementwise_min(mn, __builtin_elementwise_min(lo, hi));
+                            mx = __builtin_elementwise_max(mx, __builtin_elementwise_max(lo, hi));
+                        }
+                        const float lo = (float)min((uint32_t)mn.x, (uint32_t)mn.y);
+                        const float hi = (float)max((uint32_t)mx.x, (uint32_t)mx.y);
+                        const float dev = fmaxf(med[E][c] - lo, hi - med[E][c]);
+                        const float a = unit ? dev : wi[E][c] * dev;
+                        bound = c == 0 ? a * a : bound + a * a;
                     }
-                    const float lo = (float)min((uint32_t)mn.x, (uint32_t)mn.y);
-                    const float hi = (float)max((uint32_t)mx.x, (uint32_t)mx.y);
-                    const float dev = fmaxf(med[c] - lo, hi - med[c]);
-                    const float a = unit ? dev : wi[c] * dev;
-                    bound = c == 0 ? a * a : bound + a * a;
+                    may_outlier = bound >= tsq;
                 }
-                may_outlier = bound >= tsq;
             }
-        }
-        if (may_outlier) {
-            if (!full_tile) classify(F_{}, F_{}, T_{}, T_{}, T_{});
-            else if (want_max) {
-                if (want_sums) classify(F_{}, F_{}, T_{}, T_{}, F_{});
-                else if (unit) classify(T_{}, T_{}, T_{}, F_{}, F_{});
-                else if (pos) classify(F_{}, T_{}, T_{}, F_{}, F_{});
-                else classify(F_{}, F_{}, T_{}, F_{}, F_{});
-            } else {
-                if (want_sums) classify(F_{}, F_{}, F_{}, T_{}, F_{});
-                else if (unit) classify(T_{}, T_{}, F_{}, F_{}, F_{});
-                else if (pos) classify(F_{}, T_{}, F_{}, F_{}, F_{});
-                else classify(F_{}, F_{}, F_{}, F_{}, F_{});
+            if (may_outlier) {
+                if (!full_tile) classify(e_t, F_{}, F_{}, T_{}, T_{}, T_{});
+                else if (want_max) {
+                    if (want_sums) classify(e_t, F_{}, F_{}, T_{}, T_{}, F_{});
+                    else if (unit) classify(e_t, T_{}, T_{}, T_{}, F_{}, F_{});
+                    else if (pos) classify(e_t, F_{}, T_{}, T_{}, F_{}, F_{});
+                    else classify(e_t, F_{}, F_{}, T_{}, F_{}, F_{});
+                } else {
+                    if (want_sums) classify(e_t, F_{}, F_{}, F_{}, T_{}, F_{});
+                    else if (unit) classify(e_t, T_{}, T_{}, F_{}, F_{}, F_{});
+                    else if (pos) classify(e_t, F_{}, T_{}, F_{}, F_{}, F_{});
+                    else classify(e_t, F_{}, F_{}, F_{}, F_{}, F_{});
+                }
             }
-        }
+        });
         if (want_sums) {
             // all-frame sums through v_sad_u8 against zero; pads (255) are taken back out
             const uint32_t npad = t0 + F > N ? min(t0 + F - N, (uint32_t)F) : 0u;
 #pragma unroll
-            for (int c = 0; c < C; ++c) {
-                uint32_t acc = 0;
+            for (int e = 0; e < X; ++e)
 #pragma unroll
-                for (int i = 0; i < ND; ++i) acc = __builtin_amdgcn_sad_u8(yv[c][i], 0u, acc);
-                sum_all[c] = (uint32_t)grp_sum<G>((int32_t)(acc - npad * 255u));
-                sum_out[c] = (uint32_t)grp_sum<G>((int32_t)sum_out[c]);
-            }
+                for (int c = 0; c < C; ++c) {
+                    uint32_t acc = 0;
+#pragma unroll
+                    for (int i = 0; i < ND; ++i) acc = __builtin_amdgcn_sad_u8(yv[e * C + c][i], 0u, acc);
+                    sum_all[e][c] = (uint32_t)grp_sum<G>((int32_t)(acc - npad * 255u));
+                    sum_out[e][c] = (uint32_t)grp_sum<G>((int32_t)sum_out[e][c]);
+                }
         }
 
-        // ---- leave the lane's summaries and the pixel's statistics in LDS
+        // ---- leave the lane's summaries and the pixels' statistics in LDS
         {
             const int32_t nv = min((int32_t)N - (int32_t)t0, (int32_t)F);  // valid frames in this block
-            const uint32_t sl = TS::slot(g, pl);
 #pragma unroll
-            for (int w = 0; w < MW; ++w) {
-                const int32_t nw = nv - 32 * w;
-                const uint32_t vm = nw >= 32 ? 0xFFFFFFFFu : (nw <= 0 ? 0u : (1u << nw) - 1u);
-                sh.mask[w][sl] = m[w] & vm;
-            }
-            if (want_max) {
-                sh.lmax[sl] = lmax;
-                sh.lmaxi[sl] = lmaxi;
+            for (int e = 0; e < X; ++e) {
+                const uint32_t sl = TS::slot(g, pl * X + e);
+#pragma unroll
+                for (int w = 0; w < MW; ++w) {
+                    const int32_t nw = nv - 32 * w;
+                    const uint32_t vm = nw >= 32 ? 0xFFFFFFFFu : (nw <= 0 ? 0u : (1u << nw) - 1u);
+                    sh.mask[w][sl] = m[e][w] & vm;
+                }
+                if (want_max) {
+                    sh.lmax[sl] = lmax[e];
+                    sh.lmaxi[sl] = lmaxi[e];
+                }
             }
         }
         if (g == 0) {
             const bool want_iqr = !ABS || k.stats != nullptr;
             const bool want_s0 = k.background < 2;
 #pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                sh.med[c][pl] = med[c];
-                if (want_iqr) sh.iqr_inv[c][pl] = iqr_inv[c];
-            }
+            for (int e = 0; e < X; ++e) {
+                const uint32_t p = pl * X + e;
 #pragma unroll
-            for (int c = 0; c < C; ++c) {
-                if (want_s0) sh.s0[c][pl] = yv[c][0] & 0xFFu;
-                if (want_sums) {
-                    sh.sum_all[c][pl] = sum_all[c];
-                    sh.sum_out[c][pl] = sum_out[c];
+                for (int c = 0; c < 4; ++c) {
+                    sh.med[c][p] = med[e][c];
+                    if (want_iqr) sh.iqr_inv[c][p] = iqr_inv[e][c];
+                }
+#pragma unroll
+                for (int c = 0; c < C; ++c) {
+                    if (want_s0) sh.s0[c][p] = yv[e * C + c][0] & 0xFFu;
+                    if (want_sums) {
+                        sh.sum_all[c][p] = sum_all[e][c];
+                        sh.sum_out[c][p] = sum_out[e][c];
+                    }
                 }
             }
         }
     }
     __syncthreads();
-    for (uint32_t p = lane; p < (uint32_t)P; p += 256) finish_pixel<1, C, G, F, 1, ABS>(k, bx, y, p, sh);
+    for (uint32_t p = lane; p < (uint32_t)P; p += 256) finish_pixel<1, C, G, F, X, ABS>(k, bx, y, p, sh);
 }
 
-// VGPRs the frame blocks take, for the occupancy hint: C * (F + FS) / 4 plus working registers.
-constexpr int waves_hint_u8(int C, int F, int FS) {
-    const int w = 448 / (C * (F + FS) / 4 + 64);
+// VGPRs the frame blocks take, for the occupancy hint: X * C * (F + FS) / 4 plus working
+// registers.
+constexpr int waves_hint_u8(int C, int F, int X, int FS) {
+    const int w = 448 / (X * C * (F + FS) / 4 + 64);
     return w < 1 ? 1 : (w > 8 ? 8 : w);
 }
 
-template <int C, int G, int F, int FS, bool ABS>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(waves_hint_u8(C, F, FS))))
+template <int C, int G, int F, int X, int FS, bool ABS>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(waves_hint_u8(C, F, X, FS))))
 outlier_u8_kernel(OutlierK k, uint32_t tiles_x) {
-    __shared__ TileShared<C, G, F, 1> sh;
+    __shared__ TileShared<C, G, F, X> sh;
     __shared__ uint64_t soff[FS > 0 ? G * (FS + 2) : 1];
     const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
     load_offsets<1, F>(sh.off, G * F, k.pos_off, k.n);
     if constexpr (FS > 0) load_offsets<1, FS>(soff, G * FS, k.samp_off, k.n_sample);
     __syncthreads();
-    outlier_u8_tile<C, G, F, FS, ABS>(k, tile % tiles_x, tile / tiles_x, sh, soff);
+    outlier_u8_tile<C, G, F, X, FS, ABS>(k, tile % tiles_x, tile / tiles_x, sh, soff);
+}
+
+// Whether the packed-byte tiles may run over this composite: every lane's load of a frame
+// (X * C bytes read with up to 8 bytes around them from the dword at or below them) stays
+// inside the stack allocation, and 4-channel stacks start on a dword boundary.
+inline bool u8_tile_fits(const void* stack, uint64_t alloc_bytes, uint32_t channels, int X, const OutlierK& k,
+                         uint64_t max_pos) {
+    if (channels == 4 && ((uintptr_t)stack & 3u)) return false;
+    const uint64_t x_last = (uint64_t)((k.out_w - 1) / X) * X;
+    const uint64_t first = max_pos + ((uint64_t)(k.out_h - 1) * k.stack_w + x_last) * channels;
+    return first + (uint64_t)X * channels + 8 <= alloc_bytes;
 }
 
 }  // namespace cp

@@ -25,20 +25,24 @@ int launch_outlier(cp_ctx* ctx, const cp_stack* st, const OutlierK& k, uint64_t 
     // (N * max sample < 2^24: always for 8-bit stacks).  8-bit composites take the packed-byte
     // kernel up to 1280 frames (and, with a `--sample` subset, up to 128 sample frames) or the
     // tuned choice between the two kernels; 16-bit ones the 16-bit kernel up to 512 frames
-    // without a sample subset; the one-lane-per-pixel generic kernel takes the rest.
+    // without a sample subset; the one-lane-per-pixel generic kernel takes the rest, and the
+    // packed-byte composites whose wide loads would read past a borrowed stack's end.
     const bool sums = k.background == 2 || k.outlier == 3;
     const bool exact_sums = st->bps == 1 || k.n <= 256;
     bool done = false;
     if (!(flags & CP_FLAG_FORCE_GENERIC) && (!sums || exact_sums)) {
+        auto u8_fits = [&](int sh) {
+            return u8_tile_fits(st->dev, st->alloc_bytes, st->channels, kU8Shapes[sh].X, k, max_pos);
+        };
         if (k.sample_mask != nullptr) {
             if (st->bps == 1 && k.n <= kU8MaxFrames && k.n_sample <= kU8MaxSample && k.samp_off) {
                 const int sh = u8_shape_for(k.n, true, k.n_sample);
-                if (sh >= 0) done = launch_outlier_u8(st->channels, k, sh, s);
+                if (sh >= 0 && u8_fits(sh)) done = launch_outlier_u8(st->channels, k, sh, s);
             }
         } else {
             int choice = ctx->choice_for(st->bps, st->channels, k.n);
             if (choice >= kU8Base) {
-                done = launch_outlier_u8(st->channels, k, choice - kU8Base, s);
+                if (u8_fits(choice - kU8Base)) done = launch_outlier_u8(st->channels, k, choice - kU8Base, s);
             } else if (choice >= 0) {
                 if (kShapes[choice].X == 2 &&
                     !pair_tile_fits(st->dev, st->alloc_bytes, st->bps, st->channels, k, max_pos))

@@ -33,11 +33,11 @@ constexpr int kMaxCand = 3;
 constexpr int kBucketCand[2][2][kNumBuckets][kMaxCand] = {
     // 8-bit, 1-2 channels: the packed-byte kernel only
     {{{kU8Base + 0, -1, -1}, {kU8Base + 1, -1, -1}, {kU8Base + 2, -1, -1}, {kU8Base + 3, -1, -1},
-      {kU8Base + 4, -1, -1}, {kU8Base + 5, -1, -1}, {kU8Base + 6, -1, -1}, {kU8Base + 7, -1, -1},
-      {kU8Base + 8, -1, -1}},
+      {kU8Base + 4, -1, -1}, {kU8Base + 6, kU8Base + 5, -1}, {kU8Base + 7, -1, -1}, {kU8Base + 8, -1, -1},
+      {kU8Base + 9, -1, -1}},
      // 8-bit, 3-4 channels
      {{0, kU8Base + 0, -1}, {1, 2, kU8Base + 1}, {3, 4, kU8Base + 2}, {5, 6, kU8Base + 3}, {7, 8, kU8Base + 4},
-      {9, kU8Base + 5, -1}, {kU8Base + 6, -1, -1}, {kU8Base + 7, -1, -1}, {kU8Base + 8, -1, -1}}},
+      {9, kU8Base + 6, kU8Base + 5}, {kU8Base + 7, -1, -1}, {kU8Base + 8, -1, -1}, {kU8Base + 9, -1, -1}}},
     // 16-bit: 1-2 channels have no register-resident kernel; 3-4 channels up to 512 frames
     {{{-1, -1, -1}, {-1, -1, -1}, {-1, -1, -1}, {-1, -1, -1}, {-1, -1, -1}, {-1, -1, -1}, {-1, -1, -1},
       {-1, -1, -1}, {-1, -1, -1}},

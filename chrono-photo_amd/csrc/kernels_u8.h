@@ -1,22 +1,25 @@
 // kernels_u8.h -- tile shapes of the 8-bit register-resident outlier kernel and their launch.
 //
-// A shape (G, F, FS) puts G lanes on each pixel, F frames of all positions in each lane
-// (G * F >= N) and, for `--sample` composites, FS frames of the sample positions (G * FS >= S).
+// A shape (G, F, X, FS) puts G lanes on each group of X pixels, F frames of all positions in
+// each lane (G * F >= N) and, for `--sample` composites, FS frames of the sample positions
+// (G * FS >= S).
 #pragma once
 #include "k_outlier_u8.h"
 
 namespace cp {
 
 struct U8Shape {
-    int G, F, FS;
+    int G, F, X, FS;
 };
 
-constexpr int kNumU8Shapes = 15;
+// One pixel per lane for short stacks; four pixels per lane (one 16-byte load per frame)
+// for tall ones; `--sample` shapes carry an FS-frame block of the sample positions.
+constexpr int kNumU8Shapes = 14;
 constexpr U8Shape kU8Shapes[kNumU8Shapes] = {
-    {1, 16, 0},  {1, 32, 0},  {2, 32, 0},  {4, 32, 0},  {8, 32, 0},  {16, 32, 0}, {16, 48, 0}, {16, 64, 0},
-    {16, 80, 0}, {4, 32, 8},  {8, 32, 8},  {16, 32, 8}, {16, 48, 8}, {16, 64, 8}, {16, 80, 8}};
-constexpr uint32_t kU8MaxFrames = 16 * 80;
-constexpr uint32_t kU8MaxSample = 16 * 8;
+    {1, 16, 1, 0},  {1, 32, 1, 0},  {2, 32, 1, 0},  {4, 32, 1, 0}, {8, 32, 1, 0},  {16, 32, 1, 0}, {16, 32, 4, 0},
+    {32, 24, 4, 0}, {32, 32, 4, 0}, {32, 40, 4, 0}, {4, 32, 1, 8}, {8, 32, 1, 8}, {16, 32, 4, 8}, {32, 40, 4, 4}};
+constexpr uint32_t kU8MaxFrames = 32 * 40;
+constexpr uint32_t kU8MaxSample = 128;
 
 // The smallest shape that holds n frames (and s sample frames when sampled), or -1.
 inline int u8_shape_for(uint32_t n, bool sampled, uint32_t s) {
@@ -28,33 +31,32 @@ inline int u8_shape_for(uint32_t n, bool sampled, uint32_t s) {
     return -1;
 }
 
-template <int C, int G, int F, int FS>
+template <int C, int G, int F, int X, int FS>
 inline void launch_u8_shape(const OutlierK& k, hipStream_t s) {
-    constexpr int P = 256 / G;
+    constexpr int P = X * 256 / G;
     const uint32_t tiles_x = (k.out_w + P - 1) / P;
     dim3 grid(tiles_x * k.out_h), block(256);
-    if (k.th.absolute) hipLaunchKernelGGL((outlier_u8_kernel<C, G, F, FS, true>), grid, block, 0, s, k, tiles_x);
-    else hipLaunchKernelGGL((outlier_u8_kernel<C, G, F, FS, false>), grid, block, 0, s, k, tiles_x);
+    if (k.th.absolute) hipLaunchKernelGGL((outlier_u8_kernel<C, G, F, X, FS, true>), grid, block, 0, s, k, tiles_x);
+    else hipLaunchKernelGGL((outlier_u8_kernel<C, G, F, X, FS, false>), grid, block, 0, s, k, tiles_x);
 }
 
 template <int C>
 inline bool launch_u8_c(const OutlierK& k, int shape, hipStream_t s) {
     switch (shape) {
-    case 0: launch_u8_shape<C, 1, 16, 0>(k, s); return true;
-    case 1: launch_u8_shape<C, 1, 32, 0>(k, s); return true;
-    case 2: launch_u8_shape<C, 2, 32, 0>(k, s); return true;
-    case 3: launch_u8_shape<C, 4, 32, 0>(k, s); return true;
-    case 4: launch_u8_shape<C, 8, 32, 0>(k, s); return true;
-    case 5: launch_u8_shape<C, 16, 32, 0>(k, s); return true;
-    case 6: launch_u8_shape<C, 16, 48, 0>(k, s); return true;
-    case 7: launch_u8_shape<C, 16, 64, 0>(k, s); return true;
-    case 8: launch_u8_shape<C, 16, 80, 0>(k, s); return true;
-    case 9: launch_u8_shape<C, 4, 32, 8>(k, s); return true;
-    case 10: launch_u8_shape<C, 8, 32, 8>(k, s); return true;
-    case 11: launch_u8_shape<C, 16, 32, 8>(k, s); return true;
-    case 12: launch_u8_shape<C, 16, 48, 8>(k, s); return true;
-    case 13: launch_u8_shape<C, 16, 64, 8>(k, s); return true;
-    case 14: launch_u8_shape<C, 16, 80, 8>(k, s); return true;
+    case 0: launch_u8_shape<C, 1, 16, 1, 0>(k, s); return true;
+    case 1: launch_u8_shape<C, 1, 32, 1, 0>(k, s); return true;
+    case 2: launch_u8_shape<C, 2, 32, 1, 0>(k, s); return true;
+    case 3: launch_u8_shape<C, 4, 32, 1, 0>(k, s); return true;
+    case 4: launch_u8_shape<C, 8, 32, 1, 0>(k, s); return true;
+    case 5: launch_u8_shape<C, 16, 32, 1, 0>(k, s); return true;
+    case 6: launch_u8_shape<C, 16, 32, 4, 0>(k, s); return true;
+    case 7: launch_u8_shape<C, 32, 24, 4, 0>(k, s); return true;
+    case 8: launch_u8_shape<C, 32, 32, 4, 0>(k, s); return true;
+    case 9: launch_u8_shape<C, 32, 40, 4, 0>(k, s); return true;
+    case 10: launch_u8_shape<C, 4, 32, 1, 8>(k, s); return true;
+    case 11: launch_u8_shape<C, 8, 32, 1, 8>(k, s); return true;
+    case 12: launch_u8_shape<C, 16, 32, 4, 8>(k, s); return true;
+    case 13: launch_u8_shape<C, 32, 40, 4, 4>(k, s); return true;
     default: return false;
     }
 }

@@ -107,7 +107,7 @@ def test_batch_against_oracle(ctx):
 # groups fall back to the bucket's default shape of the batched kernel)
 CANDIDATES_U8 = [[(1, 16, 1), (1, 16, 1, 1)], [(1, 32, 1), (2, 16, 1), (1, 32, 1, 1)],
                  [(2, 32, 1), (4, 16, 1), (2, 32, 1, 1)], [(4, 32, 1), (8, 16, 1), (4, 32, 1, 1)],
-                 [(8, 32, 1), (4, 64, 1), (8, 32, 1, 1)], [(8, 64, 1), (16, 32, 1, 1)]]
+                 [(8, 32, 1), (4, 64, 1), (8, 32, 1, 1)], [(8, 64, 1), (16, 32, 4, 1), (16, 32, 1, 1)]]
 
 
 def test_batch_with_every_candidate_shape_against_oracle(ctx):
@@ -207,7 +207,7 @@ def test_autotune_keeps_results(ctx, bps):
             assert sh is None
             continue
         g, f, xp, kind = sh
-        assert g * f >= nmax and g in (1, 2, 4, 8, 16) and xp in ((1,) if bps == 1 else (1, 2))
+        assert g * f >= nmax and g in (1, 2, 4, 8, 16, 32) and xp in ((1, 4) if bps == 1 else (1, 2))
         assert kind in ((0, 1) if bps == 1 else (0,))
     for n in (12, 30, 60, 120, 250) + ((700, 1200) if bps == 1 else ()):
         x, st = scene_stack(ctx, 7 + n, n, 70, 24, 3, bps)

@@ -10,9 +10,12 @@ from test_parity_gpu import MODES, assert_same, make_stack, run_both
 
 pytestmark = pytest.mark.gpu
 
-# the packed-byte choice of each bucket (lanes per pixel, frames per lane, pixels per lane, kernel)
-U8_CHOICE = [(1, 16, 1, 1), (1, 32, 1, 1), (2, 32, 1, 1), (4, 32, 1, 1), (8, 32, 1, 1), (16, 32, 1, 1),
-             (16, 48, 1, 1), (16, 64, 1, 1), (16, 80, 1, 1)]
+# packed-byte choices of every bucket (lanes per pixel group, frames per lane, pixels per lane,
+# kernel): the two sets differ in the 512-frame bucket (four pixels or one pixel per lane)
+U8_CHOICES = [[(1, 16, 1, 1), (1, 32, 1, 1), (2, 32, 1, 1), (4, 32, 1, 1), (8, 32, 1, 1), (16, 32, 4, 1),
+               (32, 24, 4, 1), (32, 32, 4, 1), (32, 40, 4, 1)],
+              [(1, 16, 1, 1), (1, 32, 1, 1), (2, 32, 1, 1), (4, 32, 1, 1), (8, 32, 1, 1), (16, 32, 1, 1),
+               (32, 24, 4, 1), (32, 32, 4, 1), (32, 40, 4, 1)]]
 
 
 def _tall(seed, n, h, w, c, frac=0.1):
@@ -71,13 +74,14 @@ def test_every_packed_byte_shape(ctx, c):
     """The packed-byte choice pinned in every bucket, at frame counts that fill each shape and
     one frame short of it."""
     try:
-        cpa.tile_shapes_set(ctx, 1, c, U8_CHOICE)
-        for nmax in cpa.TILE_BUCKETS:
-            for n in (nmax, nmax - 1):
-                x = _tall(3700 + n + c, n, 4, 33, c, frac=0.05)
-                for absolute, lo, hi, bg, sel in ((True, 0.05, 0.2, 0, 4), (False, 2.0, 4.0, 2, 3)):
-                    gpu, ref = run_both(ctx, x, absolute, lo, hi, bg, sel)
-                    assert_same(gpu, ref, f"c={c} n={n} abs={absolute}")
+        for pick, choice in enumerate(U8_CHOICES):
+            cpa.tile_shapes_set(ctx, 1, c, choice)
+            for nmax in cpa.TILE_BUCKETS[(5 if pick else 0):(6 if pick else None)]:
+                for n in (nmax, nmax - 1):
+                    x = _tall(3700 + n + c, n, 4, 70, c, frac=0.05)
+                    for absolute, lo, hi, bg, sel in ((True, 0.05, 0.2, 0, 4), (False, 2.0, 4.0, 2, 3)):
+                        gpu, ref = run_both(ctx, x, absolute, lo, hi, bg, sel)
+                        assert_same(gpu, ref, f"c={c} n={n} pick={pick} abs={absolute}")
     finally:
         cpa.tile_shapes_set(ctx, 1, c, [])
 
@@ -98,3 +102,43 @@ def test_tall_weights_crops_and_selections(ctx):
             gpu, ref = run_both(ctx, x, absolute, lo, hi, bg, sel, weights=weights, fade=fade, slots=slots,
                                 crop=crop)
             assert_same(gpu, ref, f"weights={weights} abs={absolute} bg={bg} sel={sel}")
+
+
+@pytest.mark.parametrize("c,shift", [(3, 0), (3, 1), (4, 0), (1, 3), (2, 2)])
+def test_borrowed_stack_ending_at_its_last_frame(ctx, c, shift):
+    """A borrowed allocation ends at its last frame and may start anywhere: composites whose
+    wide loads would read past that end (or 4-channel stacks off a dword boundary) take the
+    generic kernel, the rest the packed-byte kernel; both exact."""
+    from chrono_photo_amd import hip as H
+    for n in (40, 700):
+        x = np.ascontiguousarray(_tall(4100 + n + c, n, 5, 27, c))
+        buf = H.DeviceBuffer(x.nbytes + shift)
+        H.check(H.hip().hipMemcpy(buf.ptr + shift, x.ctypes.data, x.nbytes, H.hipMemcpyHostToDevice), "hipMemcpy")
+        st = cpa.Stack(ctx, 27, 5, c, 1, device_ptr=buf.ptr + shift, n_frames=n)
+        for sel in (2, 4):
+            proc = cpa.OutlierProcessor(cpa.Threshold.abs(0.05, 0.2), 0, sel)
+            img, mask = proc.process(st)
+            oimg, omask, _ = O.outlier(x, O.threshold(True, 0.05, 0.2, 1), 0, sel)
+            assert np.array_equal(img, oimg), (c, shift, n, sel)
+            assert np.array_equal(mask, omask), (c, shift, n, sel)
+        st.close()
+        buf.free()
+
+
+@pytest.mark.parametrize("n", [300, 1100])
+def test_four_pixel_tiles_with_crops(ctx, n):
+    """Per-frame crop origins of up to +-3 pixels move every frame's 4-pixel group to every
+    byte alignment of its 16-byte load; rows with partial last tiles."""
+    rng = np.random.default_rng(n)
+    h, w = 7, 75
+    x = _tall(4300 + n, n, h, w, 3, frac=0.1)
+    offs = rng.integers(-3, 4, (n, 2)).tolist()
+    offs[0] = [0, 0]
+    crop = O.crop_create(offs, w, h)
+    try:
+        cpa.tile_shapes_set(ctx, 1, 3, U8_CHOICES[0])
+        for absolute, lo, hi, bg, sel in ((True, 0.05, 0.2, 0, 4), (False, 2.0, 4.0, 2, 3), (True, 0.03, 0.1, 1, 2)):
+            gpu, ref = run_both(ctx, x, absolute, lo, hi, bg, sel, crop=crop, seed=5)
+            assert_same(gpu, ref, f"n={n} abs={absolute} bg={bg} sel={sel}")
+    finally:
+        cpa.tile_shapes_set(ctx, 1, 3, [])
