@@ -113,7 +113,9 @@ BATCH_SETS = {
 
 def run_batching(cpa, H, ctx, kind="large"):
     """Independent RGB8 stacks of uneven sizes (abs/rel thresholds alternating, the six
-    --outlier modes round-robin): one graph of per-group launches vs one launch per stack.
+    --outlier modes round-robin): one graph replay of the batch (one launch per kernel shape the
+    jobs are planned on, covering all their tiles) vs one launch per stack, each waited for; the
+    same launches queued back to back without waits are timed too, to show what the waits cost.
     "large": 48 stacks, frames U{8..200}, 640x480..1920x1080; "small": 256 stacks, frames
     U{8..64}, 160x120..640x480."""
     import numpy as np
@@ -158,9 +160,19 @@ def run_batching(cpa, H, ctx, kind="large"):
             best = min(best, time.perf_counter() - t0)
         return best
 
+    def queued():
+        for j in jobs:
+            outs = cpa.cp_outputs(j.c.out.px, j.c.out.mask, None, None)
+            cpa._check(lib.cp_outlier(ctx.handle, j.stack.handle, C.byref(j.sel.c), C.byref(j.c.outlier),
+                                      C.byref(outs), cpa.CP_FLAG_DEVICE_OUT))
+
     plain_s = timed(plain)
+    queued_s = timed(queued)
     ref = [b[1].to_host(np.zeros(b[1].nbytes, np.uint8)) for b in bufs]
+    for b in bufs:  # the batched run must write every byte again
+        b[1].from_host(np.zeros(b[1].nbytes, np.uint8))
     batch = cpa.Batch(ctx, jobs, graph=True)
+    info = batch.info()
     batch_s = timed(lambda: batch.launch())
     same = all(np.array_equal(b[1].to_host(np.zeros(b[1].nbytes, np.uint8)), r) for b, r in zip(bufs, ref))
     batch.close()
@@ -171,10 +183,65 @@ def run_batching(cpa, H, ctx, kind="large"):
     return {"stacks": n_stacks, "frames": list(cfg["frames"]),
             "sizes": f"{cfg['width'][0]}x{cfg['height'][0]}..{cfg['width'][1]}x{cfg['height'][1]}",
             "pixel_samples": total, "plain_ms": plain_s * 1e3, "batched_ms": batch_s * 1e3,
+            "queued_ms": queued_s * 1e3,
             "plain_value": total / plain_s, "batched_value": total / batch_s,
-            "speedup": plain_s / batch_s, "byte_identical": bool(same),
+            "speedup": plain_s / batch_s, "speedup_vs_queued": queued_s / batch_s,
+            "byte_identical": bool(same), "launches": info,
             "plain": f"{n_stacks} sequential cp_outlier launches, one stack each, each waited for",
-            "batched": "one graph replay: one launch per tile-shape group (cp_batch_create/cp_batch_launch)"}
+            "queued": f"the same {n_stacks} cp_outlier launches queued back to back, one wait at the end",
+            "batched": "one graph replay: a launch per kernel shape over all its jobs' tiles, "
+                       "spread over four streams (cp_batch_create/cp_batch_launch), kernel choices "
+                       "from cp_autotune"}
+
+
+def run_workload(cpa, H, ctx, name, steps=10, warmup=2):
+    """One outlier workload without shake correction (its stack generated in HBM), timed like
+    the headline: HIP events around `steps` cp_outlier launches; its roofline from the same
+    algorithmic bytes and its committed traffic summary."""
+    import numpy as np
+    wl = WORKLOADS[name]
+    scene = cpa.Scene(**wl["scene"])
+    n, w, h, ch, bps = scene.n_frames, scene.width, scene.height, scene.channels, scene.bytes_per_sample
+    stack = _stack_from_scene(cpa, ctx, scene)
+    sel = cpa._FrameSel(stack, None, None, n)
+    kind, lo, hi = wl["threshold"]
+    sample_pos = None
+    if wl.get("sample"):
+        sample_pos = np.sort(np.random.default_rng(7).choice(n, wl["sample"], replace=False))
+    proc = cpa.OutlierProcessor(cpa.Threshold(kind == "abs", lo, hi), cpa.BackgroundMode.from_str(wl["background"]),
+                                cpa.OutlierSelectionMode.from_str(wl["outlier"]), sample_indices=sample_pos)
+    params = proc.params(bps)
+    out_px, out_mask = H.DeviceBuffer(w * h * ch * bps), H.DeviceBuffer(w * h * ch)
+    outs = cpa.cp_outputs(out_px.ptr, out_mask.ptr, None, None)
+    lib = ctx.lib
+
+    def step():
+        cpa._check(lib.cp_outlier(ctx.handle, stack.handle, C.byref(sel.c), C.byref(params), C.byref(outs),
+                                  cpa.CP_FLAG_DEVICE_OUT))
+    for _ in range(warmup):
+        step()
+    ctx.synchronize()
+    e0, e1 = H.Event(), H.Event()
+    e0.record(ctx.stream)
+    for _ in range(steps):
+        step()
+    e1.record(ctx.stream)
+    e1.synchronize()
+    ms = e0.elapsed_ms(e1) / steps
+    alg = n * w * h * ch * bps + w * h * ch * (bps + 1)
+    gbs = alg / (ms / 1e3) / 1e9
+    traffic = measured_traffic(name)
+    stack.close()
+    out_px.free()
+    out_mask.free()
+    return {"ms_per_step": ms, "value": n * w * h / (ms / 1e3), "unit": "pixel-samples/s",
+            "config": {"frames": n, "width": w, "height": h, "channels": ch, "bytes_per_sample": bps,
+                       "threshold": "/".join(map(str, wl["threshold"])), "background": wl["background"],
+                       "outlier": wl["outlier"], **({"sample": wl["sample"]} if wl.get("sample") else {})},
+            "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": gbs / HBM_PEAK_GBS, "traffic": traffic["bytes"] if traffic else None,
+                         **({"traffic_source": traffic["source"]} if traffic else {}),
+                         "algorithmic_bytes_per_launch": alg}}
 
 
 def run_serving(cpa, H, ctx, seed=4, n_jobs=24, window=60, n_video=600, load=0.7, check_every=27):
@@ -315,7 +382,16 @@ def main():
     ap.add_argument("--no-extras", action="store_true", help="skip the batching and serving measurements")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (scale factor for HBM byte counters)")
+    ap.add_argument("--batching", choices=sorted(BATCH_SETS),
+                    help="only run the autotuner and this batching set (one JSON line), e.g. for a trace")
     args = ap.parse_args()
+    if args.batching:
+        import chrono_photo_amd as cpa
+        from chrono_photo_amd import hip as H
+        ctx = cpa.Context(0)
+        cpa.autotune(ctx, 1, 3)
+        print(json.dumps(run_batching(cpa, H, ctx, args.batching)))
+        return
     if args.copy_calibration:
         from chrono_photo_amd import hip as H
         src, dst = H.DeviceBuffer(1 << 30), H.DeviceBuffer(1 << 30)
@@ -458,14 +534,33 @@ def main():
         stack.close()
         out_px.free()
         out_mask.free()
+        # the 8-bit workloads: 64-frame relative-threshold average, and the tall galleries
+        extra["workloads"] = {}
+        for name in ("outlier_1080p_rgb8_64f", "outlier_1080p_rgb8_1200f", "outlier_1080p_rgb8_1200f_s64"):
+            t0 = time.time()
+            r = extra["workloads"][name] = run_workload(cpa, H, ctx, name)
+            print(f"[bench] {name}: {r['ms_per_step']:.3f} ms/step, {r['value']:.4g} pixel-samples/s, "
+                  f"{r['roofline']['achieved']:.0f} GB/s ({100 * r['roofline']['frac']:.1f}% of HBM peak) "
+                  f"({time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
+        t0 = time.time()
+        tuned = cpa.autotune(ctx, 1, 3)  # the batching sets are 8-bit RGB
+        extra["autotune_u8_rgb"] = {kind: [None if sh is None else "x".join(map(str, sh)) for sh in v]
+                                    for kind, v in tuned.items()}
+        print(f"[bench] autotune (8-bit RGB, G x F x X x kernel per bucket): {extra['autotune_u8_rgb']} "
+              f"({time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
         t0 = time.time()
         extra["batching"] = run_batching(cpa, H, ctx, "large")
-        print(f"[bench] batching (48 large stacks): {extra['batching']['speedup']:.2f}x over plain "
+        bt = extra["batching"]
+        print(f"[bench] batching (48 large stacks): {bt['speedup']:.2f}x over plain, "
+              f"{bt['plain_ms']:.2f} -> {bt['batched_ms']:.2f} ms (queued {bt['queued_ms']:.2f} ms); "
+              f"launches {bt['launches']}; byte-identical {bt['byte_identical']} "
               f"({time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
         t0 = time.time()
         extra["batching_small"] = run_batching(cpa, H, ctx, "small")
-        print(f"[bench] batching (256 small stacks): {extra['batching_small']['speedup']:.2f}x over plain, "
-              f"{extra['batching_small']['plain_ms']:.2f} -> {extra['batching_small']['batched_ms']:.2f} ms "
+        bt = extra["batching_small"]
+        print(f"[bench] batching (256 small stacks): {bt['speedup']:.2f}x over plain, "
+              f"{bt['plain_ms']:.2f} -> {bt['batched_ms']:.2f} ms (queued {bt['queued_ms']:.2f} ms); "
+              f"launches {bt['launches']}; byte-identical {bt['byte_identical']} "
               f"({time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
         t0 = time.time()
         extra["serving"] = sv = run_serving(cpa, H, ctx)

@@ -31,8 +31,10 @@ FLAGS = [
 LIBS = {
     "libchrono_hip.so": ["abi.hip", "kernels.hip", "kernels_fast_b1_c3.hip", "kernels_fast_b1_c4.hip",
                          "kernels_fast_b2_c3.hip", "kernels_fast_b2_c4.hip", "kernels_u8_c1.hip",
-                         "kernels_u8_c2.hip", "kernels_u8_c3.hip", "kernels_u8_c4.hip", "kernels_batch_b1_c3.hip",
-                         "kernels_batch_b1_c4.hip",
+                         "kernels_u8_c2.hip", "kernels_u8_c3.hip", "kernels_u8_c4.hip", "kernels_batch_u8_c1.hip",
+                         "kernels_batch_u8_c2.hip", "kernels_batch_u8_c3.hip", "kernels_batch_u8_c4.hip",
+                         "kernels_batch_fast_b1_c3.hip", "kernels_batch_fast_b1_c4.hip",
+                         "kernels_batch_fast_b2_c3.hip", "kernels_batch_fast_b2_c4.hip",
                          "shake.hip", "batch.hip", "server.hip", "outofcore.hip"],
     "libchrono_synth.so": ["synth.hip"],
 }

@@ -126,6 +126,11 @@ class cp_server_stats(C.Structure):
                 ("kernel_downloads", C.c_uint32)]
 
 
+class cp_batch_info(C.Structure):
+    _fields_ = [("group_launches", C.c_uint32), ("group_jobs", C.c_uint32),
+                ("direct_jobs", C.c_uint32), ("reserved", C.c_uint32)]
+
+
 class cs_scene(C.Structure):
     _fields_ = [("seed", C.c_uint64), ("width", C.c_uint32), ("height", C.c_uint32),
                 ("channels", C.c_uint32), ("bytes_per_sample", C.c_uint32), ("n_frames", C.c_uint32),
@@ -145,7 +150,7 @@ ABI_SYMBOLS = [
     "cp_fade_table", "cp_fade_get", "cp_outlier", "cp_outlier_host", "cp_simple", "cp_simple_host", "cp_shake",
     "cp_shake_noalloc", "cp_crop_create",
     "cp_video_windows", "cp_batch_run", "cp_batch_create", "cp_batch_launch", "cp_batch_destroy",
-    "cp_autotune", "cp_tile_shapes_set", "cp_server_create", "cp_server_submit", "cp_server_poll",
+    "cp_batch_info_get", "cp_autotune", "cp_tile_shapes_set", "cp_server_create", "cp_server_submit", "cp_server_poll",
     "cp_server_wait", "cp_server_stats_get", "cp_server_destroy",
 ]
 
@@ -211,6 +216,7 @@ def load_library() -> C.CDLL:
         "cp_batch_create": (C.c_int, [C.c_void_p, P(cp_job), C.c_uint32, C.c_uint32, P(C.c_void_p)]),
         "cp_batch_launch": (C.c_int, [C.c_void_p]),
         "cp_batch_destroy": (C.c_int, [C.c_void_p]),
+        "cp_batch_info_get": (C.c_int, [C.c_void_p, P(cp_batch_info)]),
         "cp_autotune": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, P(C.c_int32)]),
         "cp_tile_shapes_set": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, P(C.c_int32)]),
         "cp_server_create": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, P(C.c_void_p)]),
@@ -862,7 +868,10 @@ def _job_array(jobs):
 
 
 class Batch:
-    """cp_batch: many composites in a few launches (one per tile-shape group), captured in a HIP graph."""
+    """cp_batch: many composites, one launch per register-resident kernel shape the jobs are
+    planned on (8-bit 1-4 channels up to 1280 frames, 16-bit 3-4 channels up to 512) covering
+    all their tiles, the other jobs launched one by one, spread over four streams and captured
+    in a HIP graph."""
 
     def __init__(self, ctx: Context, jobs, graph: bool = True):
         self.ctx = ctx
@@ -878,6 +887,12 @@ class Batch:
 
     def launch(self):
         _check(self.lib.cp_batch_launch(self.handle))
+
+    def info(self) -> dict:
+        i = cp_batch_info()
+        _check(self.lib.cp_batch_info_get(self.handle, C.byref(i)))
+        return {"group_launches": i.group_launches, "group_jobs": i.group_jobs,
+                "direct_jobs": i.direct_jobs}
 
     def close(self):
         if self.handle and self.ctx.handle:
@@ -907,23 +922,29 @@ def _encode_shape(sh):
     return (kind << 24) | (x << 16) | (g << 8) | f
 
 
-def tile_shapes_set(ctx: Context, bytes_per_sample: int, channels: int, shapes):
-    """cp_tile_shapes_set: pins the kernel choice per frame bucket (TILE_BUCKETS); each entry
-    is (lanes_per_pixel, frames_per_lane, pixels_per_lane[, kernel]) as autotune returns it
-    (kernel 1: the 8-bit packed-byte kernel, 0 or absent: the 16-bit-halves kernel), or None
-    for the bucket's default.  Missing trailing buckets take their defaults."""
-    shapes = list(shapes) + [None] * (len(TILE_BUCKETS) - len(shapes))
-    arr = (C.c_int32 * len(TILE_BUCKETS))(*[_encode_shape(sh) for sh in shapes])
+def tile_shapes_set(ctx: Context, bytes_per_sample: int, channels: int, shapes, rel_shapes=None):
+    """cp_tile_shapes_set: pins the kernel choice per frame bucket (TILE_BUCKETS) for absolute
+    thresholds (`shapes`) and relative ones (`rel_shapes`, default: the same); each entry is
+    (lanes_per_pixel, frames_per_lane, pixels_per_lane[, kernel]) as autotune returns it
+    (kernel 1: the 8-bit packed-byte kernel with the byte search, 2: the same with a sorting
+    network for the ranks, 0 or absent: the 16-bit-halves kernel), or None for the bucket's
+    default.  Missing trailing buckets take their defaults."""
+    nb = len(TILE_BUCKETS)
+    shapes = list(shapes) + [None] * (nb - len(shapes))
+    rel = shapes if rel_shapes is None else list(rel_shapes) + [None] * (nb - len(rel_shapes))
+    arr = (C.c_int32 * (2 * nb))(*[_encode_shape(sh) for sh in shapes + rel])
     _check(ctx.lib.cp_tile_shapes_set(ctx.handle, bytes_per_sample, channels, arr))
 
 
 def autotune(ctx: Context, bytes_per_sample: int, channels: int):
-    """Runs the kernel autotuner; returns one (lanes_per_pixel, frames_per_lane,
-    pixels_per_lane, kernel) per bucket of TILE_BUCKETS, or None where no register-resident
-    kernel covers the bucket."""
-    out = (C.c_int32 * len(TILE_BUCKETS))()
+    """Runs the kernel autotuner; returns {"abs": [...], "rel": [...]}: per bucket of
+    TILE_BUCKETS and threshold kind, the chosen (lanes_per_pixel, frames_per_lane,
+    pixels_per_lane, kernel), or None where no register-resident kernel covers the bucket."""
+    nb = len(TILE_BUCKETS)
+    out = (C.c_int32 * (2 * nb))()
     _check(ctx.lib.cp_autotune(ctx.handle, bytes_per_sample, channels, out))
-    return [None if v == 0 else ((v >> 8) & 0xFF, v & 0xFF, (v >> 16) & 0xFF, v >> 24) for v in out]
+    dec = [None if v == 0 else ((v >> 8) & 0xFF, v & 0xFF, (v >> 16) & 0xFF, v >> 24) for v in out]
+    return {"abs": dec[:nb], "rel": dec[nb:]}
 
 
 # -------------------------------------------------------------------- serving

@@ -214,10 +214,10 @@ int cp_ctx::ensure(cp::DeviceBuffer& b, size_t bytes) {
     return CP_OK;
 }
 
-int cp_ctx::choice_for(uint32_t bps, uint32_t channels, uint32_t n) const {
+int cp_ctx::choice_for(uint32_t bps, uint32_t channels, uint32_t n, bool absolute) const {
     const int b = cp::bucket_for(n);
     if (b < 0 || bps < 1 || bps > 2 || channels < 1 || channels > 4) return -1;
-    const int t = tuned[bps - 1][channels - 1][b];
+    const int t = tuned[bps - 1][channels - 1][absolute ? 0 : 1][b];
     return t >= 0 ? t : cp::kBucketCand[bps - 1][channels >= 3][b][0];
 }
 

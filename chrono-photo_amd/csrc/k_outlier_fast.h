@@ -752,10 +752,10 @@ __device__ __forceinline__ void finish_pixel(const OutlierK& k, uint32_t bx, uin
 // (load_offsets) and be visible to all lanes.  Every lane of the workgroup must call this
 // (it synchronises once).  With X = 2 the lanes read up to 16 bytes past their pixels'
 // samples (load12_a2, and pixel out_w of a row when out_w is odd); the launcher checks that
-// those bytes lie inside the stack allocation.
+// those bytes lie inside the stack allocation.  lane: threadIdx.x.
 template <int B, int C, int G, int F, int X, bool ABS>
 __device__ __forceinline__ void outlier_tile(const OutlierK& k, uint32_t bx, uint32_t y,
-                                             TileShared<C, G, F, X>& sh) {
+                                             TileShared<C, G, F, X>& sh, uint32_t lane) {
     static_assert(F % 4 == 0 && F <= 64 && G <= 8 && (X == 1 || X == 2), "tile shape");
     using TS = TileShared<C, G, F, X>;
     constexpr int P = TS::P;
@@ -765,7 +765,6 @@ __device__ __forceinline__ void outlier_tile(const OutlierK& k, uint32_t bx, uin
     constexpr uint32_t NONE = 0xFFFFFFFFu;
     const uint32_t N = k.n;
 
-    const uint32_t lane = threadIdx.x;
     const uint32_t g = lane % G;
     const uint32_t t0 = g * F;
     const int32_t ntot = G * F;
@@ -1021,7 +1020,7 @@ outlier_fast_kernel(OutlierK k, uint32_t tiles_x) {
         load_offsets<B, F>(sh.off, G * F, k.pos_off, k.n);
     }
     __syncthreads();
-    outlier_tile<B, C, G, F, X, ABS>(k, tile % tiles_x, y, sh);
+    outlier_tile<B, C, G, F, X, ABS>(k, tile % tiles_x, y, sh, threadIdx.x);
 }
 
 }  // namespace cp

@@ -27,6 +27,8 @@
 // per-pixel finish (finish_pixel, shared with the 16-bit kernel) selects the background and
 // blends (src/chrono.rs:290-493).
 #pragma once
+#include <utility>
+
 #include "k_outlier_fast.h"
 
 namespace cp {
@@ -181,13 +183,123 @@ __device__ __forceinline__ void ranks_u8(const uint32_t (&y)[C][ND], const RankP
     }
 }
 
+// ---- sorting-network ranks (one lane per pixel)
+//
+// With G = 1 a lane holds every frame of its pixel, so the ranks can come from sorting instead
+// of the byte search: Batcher's odd-even merge sort of the F frames (F a power of two),
+// two channels at a time in the 16-bit halves of a dword (v_pk_min_u16 / v_pk_max_u16 per
+// comparator), then the needed ranks picked from the sorted registers.  Any exact sort gives
+// the reference's values (src/chrono.rs:228-255 sorts too); pads (255) sort to the top.  It
+// does every rank in one pass, which pays off against three byte searches for relative
+// thresholds; the autotuner measures both per frame count.
+struct CxPair {
+    int a, b;
+};
+
+template <int N>
+constexpr int oems_count() {
+    int c = 0;
+    for (int p = 1; p < N; p <<= 1)
+        for (int k = p; k >= 1; k >>= 1)
+            for (int j = k % p; j + k < N; j += 2 * k)
+                for (int i = 0; i < k && i + j + k < N; ++i)
+                    if ((i + j) / (2 * p) == (i + j + k) / (2 * p)) ++c;
+    return c;
+}
+
+template <int N>
+struct OemsNet {
+    static constexpr int M = oems_count<N>();
+    CxPair p[M];
+    constexpr OemsNet() : p{} {
+        int c = 0;
+        for (int q = 1; q < N; q <<= 1)
+            for (int k = q; k >= 1; k >>= 1)
+                for (int j = k % q; j + k < N; j += 2 * k)
+                    for (int i = 0; i < k && i + j + k < N; ++i)
+                        if ((i + j) / (2 * q) == (i + j + k) / (2 * q)) p[c++] = CxPair{i + j, i + j + k};
+    }
+};
+
+template <int N>
+constexpr OemsNet<N> kOemsNet{};
+
+template <int A, int B, int N>
+__device__ __forceinline__ void cx_pair(us2 (&v)[N]) {
+    const us2 lo = __builtin_elementwise_min(v[A], v[B]), hi = __builtin_elementwise_max(v[A], v[B]);
+    v[A] = lo;
+    v[B] = hi;
+}
+
+template <int N, size_t... Is>
+__device__ __forceinline__ void oems_apply(us2 (&v)[N], std::index_sequence<Is...>) {
+    (cx_pair<kOemsNet<N>.p[Is].a, kOemsNet<N>.p[Is].b, N>(v), ...);
+}
+
+// v[r] for a wave-uniform rank r: the registers as vectors of up to 32 read at a dynamic
+// index, which compiles to register-relative moves (a chain of selects is turned into a copy
+// of v through scratch memory and an indexed load, and so is one read of a longer vector).
+template <int N>
+__device__ __forceinline__ us2 pick_rank(const us2 (&v)[N], uint32_t r) {
+    constexpr int W = N < 32 ? N : 32;
+    typedef uint32_t vecW __attribute__((ext_vector_type(W)));
+    const uint32_t ru = __builtin_amdgcn_readfirstlane(r);
+    uint32_t s = 0;
+#pragma unroll
+    for (int b = 0; b < N; b += W) {
+        vecW w;
+#pragma unroll
+        for (int j = 0; j < W; ++j) w[j] = __builtin_bit_cast(uint32_t, v[b + j]);
+        const uint32_t e = w[ru % W];
+        s = ru / W == (uint32_t)(b / W) ? e : s;
+    }
+    return as_us2(s);
+}
+
+// The rank values of ranks_u8 from a sort of the lane's F frames (G = 1).
+template <int F, int C, bool ABS>
+__device__ __forceinline__ void ranks_sort(const uint32_t (&y)[C][F / 4], const RankPlan& plan, uint32_t (&m0)[C],
+                                           uint32_t (&m1)[C], uint32_t (&a0)[C], uint32_t (&a1)[C],
+                                           uint32_t (&b0)[C], uint32_t (&b1)[C]) {
+    static_assert((F & (F - 1)) == 0, "the network sorts a power of two");
+    constexpr int NP = (C + 1) / 2;
+#pragma unroll
+    for (int q = 0; q < NP; ++q) {
+        const int c0 = 2 * q, c1 = 2 * q + 1 < C ? 2 * q + 1 : 2 * q;
+        us2 v[F];
+#pragma unroll
+        for (int j = 0; j < F; ++j)  // byte j&3 of each channel's dword into the two halves
+            v[j] = as_us2(__builtin_amdgcn_perm(y[c1][j >> 2], y[c0][j >> 2],
+                                                psel(j & 3, 12, 4 + (j & 3), 12)));
+        oems_apply<F>(v, std::make_index_sequence<kOemsNet<F>.M>{});
+        auto put = [&](uint32_t (&dst)[C], uint32_t r) {
+            const us2 s = pick_rank<F>(v, r);
+            dst[c0] = s.x;
+            if (c1 != c0) dst[c1] = s.y;
+        };
+        put(m0, plan.m0);
+        put(m1, plan.m1);
+        if constexpr (!ABS) {
+            put(a0, plan.a0);
+            put(a1, plan.a1);
+            put(b0, plan.b0);
+            put(b1, plan.b1);
+        }
+    }
+    if constexpr (ABS) {
+#pragma unroll
+        for (int c = 0; c < C; ++c) a0[c] = a1[c] = b0[c] = b1[c] = 0;
+    }
+}
+
 // One tile: P = X * 256 / G pixels of row y from pixel bx * P.  sh.off holds the tile's frame
 // offsets (all positions) and soff the sample positions' offsets when FS > 0; every lane of
-// the workgroup calls this (it synchronises once).
-template <int C, int G, int F, int X, int FS, bool ABS>
+// the workgroup calls this (it synchronises once).  lane: threadIdx.x.
+template <int C, int G, int F, int X, int FS, bool ABS, bool SORT = false>
 __device__ __forceinline__ void outlier_u8_tile(const OutlierK& k, uint32_t bx, uint32_t y,
-                                                TileShared<C, G, F, X>& sh, const uint64_t* soff) {
+                                                TileShared<C, G, F, X>& sh, const uint64_t* soff, uint32_t lane) {
     static_assert(F % 8 == 0 && F <= 96 && G <= 32 && FS % 4 == 0 && (X == 1 || X == 4), "tile shape");
+    static_assert(!SORT || (G == 1 && X == 1 && FS == 0), "sorting-network ranks: one lane per pixel");
     using TS = TileShared<C, G, F, X>;
     constexpr int P = TS::P;
     constexpr int MW = TS::MW;
@@ -195,7 +307,6 @@ __device__ __forceinline__ void outlier_u8_tile(const OutlierK& k, uint32_t bx, 
     constexpr int XC = X * C;
     constexpr uint32_t NONE = 0xFFFFFFFFu;
     const uint32_t N = k.n;
-    const uint32_t lane = threadIdx.x;
     const uint32_t g = lane % G;
     const uint32_t t0 = g * F;
     {
@@ -210,7 +321,9 @@ __device__ __forceinline__ void outlier_u8_tile(const OutlierK& k, uint32_t bx, 
         // ---- medians / inverse IQR over the sample set, every pixel and channel
         const RankPlan plan = rank_plan_core(k.n_sample, ABS);
         uint32_t m0[XC], m1[XC], a0[XC], a1[XC], b0[XC], b1[XC];
-        if constexpr (FS > 0) {
+        if constexpr (SORT) {
+            ranks_sort<F, XC, ABS>(yv, plan, m0, m1, a0, a1, b0, b1);
+        } else if constexpr (FS > 0) {
             constexpr int NDS = FS / 4;
             uint32_t ys[XC][NDS];
             load_block_u8<C, X, FS>(base, soff + g * (FS + 2), ys, g * FS, k.n_sample,
@@ -407,15 +520,15 @@ __device__ __forceinline__ void outlier_u8_tile(const OutlierK& k, uint32_t bx, 
     for (uint32_t p = lane; p < (uint32_t)P; p += 256) finish_pixel<1, C, G, F, X, ABS>(k, bx, y, p, sh);
 }
 
-// VGPRs the frame blocks take, for the occupancy hint: X * C * (F + FS) / 4 plus working
-// registers.
-constexpr int waves_hint_u8(int C, int F, int X, int FS) {
-    const int w = 448 / (X * C * (F + FS) / 4 + 64);
+// VGPRs the frame blocks take, for the occupancy hint: X * C * (F + FS) / 4, plus F for the
+// pairs a sorting network works on, plus working registers.
+constexpr int waves_hint_u8(int C, int F, int X, int FS, bool sort) {
+    const int w = 448 / (X * C * (F + FS) / 4 + (sort ? F : 0) + 64);
     return w < 1 ? 1 : (w > 8 ? 8 : w);
 }
 
-template <int C, int G, int F, int X, int FS, bool ABS>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(waves_hint_u8(C, F, X, FS))))
+template <int C, int G, int F, int X, int FS, bool ABS, bool SORT = false>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(waves_hint_u8(C, F, X, FS, SORT))))
 outlier_u8_kernel(OutlierK k, uint32_t tiles_x) {
     __shared__ TileShared<C, G, F, X> sh;
     __shared__ uint64_t soff[FS > 0 ? G * (FS + 2) : 1];
@@ -423,7 +536,7 @@ outlier_u8_kernel(OutlierK k, uint32_t tiles_x) {
     load_offsets<1, F>(sh.off, G * F, k.pos_off, k.n);
     if constexpr (FS > 0) load_offsets<1, FS>(soff, G * FS, k.samp_off, k.n_sample);
     __syncthreads();
-    outlier_u8_tile<C, G, F, X, FS, ABS>(k, tile % tiles_x, tile / tiles_x, sh, soff);
+    outlier_u8_tile<C, G, F, X, FS, ABS, SORT>(k, tile % tiles_x, tile / tiles_x, sh, soff, threadIdx.x);
 }
 
 // Whether the packed-byte tiles may run over this composite: every lane's load of a frame

@@ -19,38 +19,44 @@ static int launch_outlier_generic_b(const cp_stack* st, const OutlierK& k, hipSt
     return CP_OK;
 }
 
-int launch_outlier(cp_ctx* ctx, const cp_stack* st, const OutlierK& k, uint64_t max_pos, uint32_t flags,
-                   hipStream_t s) {
+OutlierPlan plan_outlier(cp_ctx* ctx, const cp_stack* st, const OutlierK& k, uint64_t max_pos, uint32_t flags) {
     // The register-resident kernels need exact integer sums for the average modes
     // (N * max sample < 2^24: always for 8-bit stacks).  8-bit composites take the packed-byte
     // kernel up to 1280 frames (and, with a `--sample` subset, up to 128 sample frames) or the
     // tuned choice between the two kernels; 16-bit ones the 16-bit kernel up to 512 frames
     // without a sample subset; the one-lane-per-pixel generic kernel takes the rest, and the
     // packed-byte composites whose wide loads would read past a borrowed stack's end.
+    OutlierPlan p;
     const bool sums = k.background == 2 || k.outlier == 3;
     const bool exact_sums = st->bps == 1 || k.n <= 256;
-    bool done = false;
-    if (!(flags & CP_FLAG_FORCE_GENERIC) && (!sums || exact_sums)) {
-        auto u8_fits = [&](int sh) {
-            return u8_tile_fits(st->dev, st->alloc_bytes, st->channels, kU8Shapes[sh].X, k, max_pos);
-        };
-        if (k.sample_mask != nullptr) {
-            if (st->bps == 1 && k.n <= kU8MaxFrames && k.n_sample <= kU8MaxSample && k.samp_off) {
-                const int sh = u8_shape_for(k.n, true, k.n_sample);
-                if (sh >= 0 && u8_fits(sh)) done = launch_outlier_u8(st->channels, k, sh, s);
-            }
-        } else {
-            int choice = ctx->choice_for(st->bps, st->channels, k.n);
-            if (choice >= kU8Base) {
-                if (u8_fits(choice - kU8Base)) done = launch_outlier_u8(st->channels, k, choice - kU8Base, s);
-            } else if (choice >= 0) {
-                if (kShapes[choice].X == 2 &&
-                    !pair_tile_fits(st->dev, st->alloc_bytes, st->bps, st->channels, k, max_pos))
-                    choice = narrow_shape(choice);
-                if (choice >= 0) done = launch_outlier_fast(st->bps, st->channels, k, choice, s);
-            }
+    if ((flags & CP_FLAG_FORCE_GENERIC) || (sums && !exact_sums)) return p;
+    auto u8_fits = [&](int sh) {
+        return u8_tile_fits(st->dev, st->alloc_bytes, st->channels, kU8Shapes[sh].X, k, max_pos);
+    };
+    if (k.sample_mask != nullptr) {
+        if (st->bps == 1 && k.n <= kU8MaxFrames && k.n_sample <= kU8MaxSample && k.samp_off) {
+            const int sh = u8_shape_for(k.n, true, k.n_sample);
+            if (sh >= 0 && u8_fits(sh)) p = {kPlanU8, sh};
         }
+        return p;
     }
+    int choice = ctx->choice_for(st->bps, st->channels, k.n, k.th.absolute != 0);
+    if (choice >= kU8Base) {
+        if (u8_fits(choice - kU8Base)) p = {kPlanU8, choice - kU8Base};
+    } else if (choice >= 0) {
+        if (kShapes[choice].X == 2 && !pair_tile_fits(st->dev, st->alloc_bytes, st->bps, st->channels, k, max_pos))
+            choice = narrow_shape(choice);
+        if (choice >= 0) p = {kPlanFast, choice};
+    }
+    return p;
+}
+
+int launch_outlier(cp_ctx* ctx, const cp_stack* st, const OutlierK& k, uint64_t max_pos, uint32_t flags,
+                   hipStream_t s) {
+    const OutlierPlan p = plan_outlier(ctx, st, k, max_pos, flags);
+    bool done = false;
+    if (p.family == kPlanU8) done = launch_outlier_u8(st->channels, k, p.shape, s);
+    else if (p.family == kPlanFast) done = launch_outlier_fast(st->bps, st->channels, k, p.shape, s);
     if (!done) {
         if (st->bps == 1) launch_outlier_generic_b<1>(st, k, s);
         else launch_outlier_generic_b<2>(st, k, s);

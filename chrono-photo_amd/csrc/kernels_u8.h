@@ -9,15 +9,19 @@
 namespace cp {
 
 struct U8Shape {
-    int G, F, X, FS;
+    int G, F, X, FS, S;  // S: ranks from a sorting network (G = 1) instead of the byte search
 };
 
 // One pixel per lane for short stacks; four pixels per lane (one 16-byte load per frame)
-// for tall ones; `--sample` shapes carry an FS-frame block of the sample positions.
-constexpr int kNumU8Shapes = 14;
+// for tall ones; `--sample` shapes carry an FS-frame block of the sample positions; the
+// last two rank one lane's whole pixel with a sorting network (at 64 frames the network's
+// comparators outnumber the byte search's steps, and its registers spill).
+constexpr int kNumU8Shapes = 16;
 constexpr U8Shape kU8Shapes[kNumU8Shapes] = {
-    {1, 16, 1, 0},  {1, 32, 1, 0},  {2, 32, 1, 0},  {4, 32, 1, 0}, {8, 32, 1, 0},  {16, 32, 1, 0}, {16, 32, 4, 0},
-    {32, 24, 4, 0}, {32, 32, 4, 0}, {32, 40, 4, 0}, {4, 32, 1, 8}, {8, 32, 1, 8}, {16, 32, 4, 8}, {32, 40, 4, 4}};
+    {1, 16, 1, 0, 0},  {1, 32, 1, 0, 0},  {2, 32, 1, 0, 0},  {4, 32, 1, 0, 0}, {8, 32, 1, 0, 0},
+    {16, 32, 1, 0, 0}, {16, 32, 4, 0, 0}, {32, 24, 4, 0, 0}, {32, 32, 4, 0, 0}, {32, 40, 4, 0, 0},
+    {4, 32, 1, 8, 0},  {8, 32, 1, 8, 0},  {16, 32, 4, 8, 0}, {32, 40, 4, 4, 0}, {1, 16, 1, 0, 1},
+    {1, 32, 1, 0, 1}};
 constexpr uint32_t kU8MaxFrames = 32 * 40;
 constexpr uint32_t kU8MaxSample = 128;
 
@@ -25,19 +29,21 @@ constexpr uint32_t kU8MaxSample = 128;
 inline int u8_shape_for(uint32_t n, bool sampled, uint32_t s) {
     for (int i = 0; i < kNumU8Shapes; ++i) {
         const U8Shape& sh = kU8Shapes[i];
-        if ((sh.FS > 0) != sampled) continue;
+        if ((sh.FS > 0) != sampled || sh.S) continue;
         if ((uint32_t)(sh.G * sh.F) >= n && (!sampled || (uint32_t)(sh.G * sh.FS) >= s)) return i;
     }
     return -1;
 }
 
-template <int C, int G, int F, int X, int FS>
+template <int C, int G, int F, int X, int FS, bool SORT = false>
 inline void launch_u8_shape(const OutlierK& k, hipStream_t s) {
     constexpr int P = X * 256 / G;
     const uint32_t tiles_x = (k.out_w + P - 1) / P;
     dim3 grid(tiles_x * k.out_h), block(256);
-    if (k.th.absolute) hipLaunchKernelGGL((outlier_u8_kernel<C, G, F, X, FS, true>), grid, block, 0, s, k, tiles_x);
-    else hipLaunchKernelGGL((outlier_u8_kernel<C, G, F, X, FS, false>), grid, block, 0, s, k, tiles_x);
+    if (k.th.absolute)
+        hipLaunchKernelGGL((outlier_u8_kernel<C, G, F, X, FS, true, SORT>), grid, block, 0, s, k, tiles_x);
+    else
+        hipLaunchKernelGGL((outlier_u8_kernel<C, G, F, X, FS, false, SORT>), grid, block, 0, s, k, tiles_x);
 }
 
 template <int C>
@@ -57,6 +63,8 @@ inline bool launch_u8_c(const OutlierK& k, int shape, hipStream_t s) {
     case 11: launch_u8_shape<C, 8, 32, 1, 8>(k, s); return true;
     case 12: launch_u8_shape<C, 16, 32, 4, 8>(k, s); return true;
     case 13: launch_u8_shape<C, 32, 40, 4, 4>(k, s); return true;
+    case 14: launch_u8_shape<C, 1, 16, 1, 0, true>(k, s); return true;
+    case 15: launch_u8_shape<C, 1, 32, 1, 0, true>(k, s); return true;
     default: return false;
     }
 }

@@ -67,10 +67,11 @@ struct cp_ctx {
     cp::ParamArena arena;
     cp::DeviceBuffer out_px, out_mask, out_stats, warn;  // host-output staging
     std::mutex mu;
-    // autotuned kernel choice per (sample bytes - 1, channels - 1, frame bucket); -1 = default
-    int8_t tuned[2][4][16];
+    // autotuned kernel choice per (sample bytes - 1, channels - 1, threshold kind (0: absolute,
+    // 1: relative), frame bucket); -1 = default
+    int8_t tuned[2][4][2][16];
     // the register-resident kernel choice for n frames (kernels_fast.h), or -1: none
-    int choice_for(uint32_t bps, uint32_t channels, uint32_t n) const;
+    int choice_for(uint32_t bps, uint32_t channels, uint32_t n, bool absolute) const;
     int ensure(cp::DeviceBuffer& b, size_t bytes);
     int charge(uint64_t bytes);
     void uncharge(uint64_t bytes) { used = used >= bytes ? used - bytes : 0; }
@@ -120,8 +121,18 @@ int stage_outlier(ParamArena& arena, const OutlierPrep& prep, const std::vector<
                   OutlierK* k);
 FadeK fade_to_device(cp_ctx* ctx, const cp_fade* f, int* rc);
 
-// Kernel launchers (defined in the kernel translation units).
+// The kernel an outlier composite runs on: the one-lane-per-pixel generic kernel, a shape of
+// the register-resident 16-bit-halves kernel (kShapes index) or of the packed-byte kernel
+// (kU8Shapes index).
+enum PlanFamily { kPlanGeneric = 0, kPlanFast = 1, kPlanU8 = 2 };
+struct OutlierPlan {
+    int family = kPlanGeneric;
+    int shape = -1;
+};
 // max_pos: the largest element offset in the composite's frame table.
+OutlierPlan plan_outlier(cp_ctx* ctx, const cp_stack* st, const OutlierK& k, uint64_t max_pos, uint32_t flags);
+
+// Kernel launchers (defined in the kernel translation units).
 int launch_outlier(cp_ctx* ctx, const cp_stack* st, const OutlierK& k, uint64_t max_pos, uint32_t flags,
                    hipStream_t s);
 int launch_simple(cp_ctx* ctx, const cp_stack* st, const SimpleK& k, hipStream_t s);

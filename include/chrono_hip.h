@@ -239,31 +239,48 @@ typedef struct {
 typedef struct cp_batch cp_batch;
 
 /* Validate a list of jobs, stage all their per-call tables in device memory owned by the
- * batch, and capture the launch sequence in a HIP graph: 8-bit outlier jobs with up to 256
- * frames, 3-4 channels and no sample subset are grouped by tile shape and threshold kind,
- * one launch per group covering all its jobs' tiles; every other job runs as its own
- * launch.  The caller's arrays are copied; output pointers must stay valid while the batch
+ * batch, and capture the launch sequence in a HIP graph.  Each outlier job is planned as its
+ * own cp_outlier call would run it (the context's tuned kernel and tile shape for its sample
+ * width, channels, frame count and threshold kind).  Jobs planned on the same register-
+ * resident kernel shape -- 8-bit stacks with 1-4 channels up to 1280 frames, with or without a
+ * sample subset, 16-bit 3-4 channel stacks up to 512 frames -- share one launch whose grid
+ * covers all their tiles, biggest jobs first.  Jobs left to the one-lane-per-pixel kernel and
+ * darker/lighter jobs are launched one by one.  The launches are spread, biggest first, over
+ * the context stream and three side streams, so the graph's branches run side by side.  The
+ * caller's arrays are copied; stacks and output pointers must stay valid while the batch
  * exists. */
 CP_API int cp_batch_create(cp_ctx* ctx, const cp_job* jobs, uint32_t n_jobs, uint32_t flags,
                            cp_batch** out);
 /* Replay the batch on the context stream (asynchronous). */
 CP_API int cp_batch_launch(cp_batch* batch);
 CP_API int cp_batch_destroy(cp_batch* batch);
+/* How a batch launches its jobs: group launches (one per kernel shape), the jobs they cover,
+ * and the jobs launched one by one. */
+typedef struct {
+    uint32_t group_launches;
+    uint32_t group_jobs;
+    uint32_t direct_jobs;
+    uint32_t reserved;
+} cp_batch_info;
+CP_API int cp_batch_info_get(const cp_batch* batch, cp_batch_info* out);
 /* create + launch + wait + destroy. */
 CP_API int cp_batch_run(cp_ctx* ctx, const cp_job* jobs, uint32_t n_jobs, uint32_t flags);
 #define CP_FLAG_NO_GRAPH 4u /* cp_batch_create: launch kernels directly, no graph capture */
 
-/* Autotuner: for each frame-count bucket of the register-resident outlier kernels (<= 16, 32,
- * 64, 128, 256, 512, 768, 1024, 1280 frames: CP_TILE_BUCKETS entries) time the candidate
- * kernel choices on a synthetic stack of this sample width and channel count (1-4) and keep
- * the fastest for the context.  A choice is a tile shape of the 16-bit-halves kernel or of
- * the 8-bit packed-byte kernel; shapes (nullable, CP_TILE_BUCKETS entries) receives
+/* Autotuner: for each threshold kind (absolute, relative) and frame-count bucket of the
+ * register-resident outlier kernels (<= 16, 32, 64, 128, 256, 512, 768, 1024, 1280 frames:
+ * CP_TILE_BUCKETS buckets) time the candidate kernel choices on a synthetic stack of this
+ * sample width and channel count (1-4) and keep the fastest for the context.  A choice is a tile shape of the 16-bit-halves kernel or of
+ * the 8-bit packed-byte kernel, whose one-lane-per-pixel shapes rank either by the byte
+ * search or by a sorting network; shapes (nullable, 2 * CP_TILE_BUCKETS entries: absolute
+ * thresholds' buckets, then relative thresholds') receives
  * (kernel << 24) | (pixels_per_lane << 16) | (lanes_per_pixel << 8) | frames_per_lane per
- * bucket (kernel 1: packed-byte), or 0 where no register-resident kernel covers the bucket. */
+ * bucket (kernel 0: 16-bit halves, 1: packed bytes with the byte search, 2: packed bytes
+ * with the sorting network), or 0 where no register-resident kernel covers the bucket. */
 #define CP_TILE_BUCKETS 9
 CP_API int cp_autotune(cp_ctx* ctx, uint32_t bytes_per_sample, uint32_t channels, int32_t* shapes);
 /* Sets the context's kernel choice per bucket without timing anything (no reference
- * counterpart): shapes holds CP_TILE_BUCKETS entries in cp_autotune's encoding, for instance
+ * counterpart): shapes holds 2 * CP_TILE_BUCKETS entries in cp_autotune's encoding, for instance
  * a result that cp_autotune returned in an earlier process, or 0 for the built-in default of
  * that bucket.  Each entry must be one of the bucket's candidates (CP_ERR_INVALID_ARG
  * otherwise, and nothing is changed). */

@@ -1,6 +1,6 @@
-"""Batched launches (one kernel per tile-shape group over many stacks, captured in a HIP graph) give
-byte-identical results to one launch per stack, and the autotuner's tile choice never
-changes results."""
+"""Batched launches (one launch per tile shape over the tiles of many stacks, captured in a HIP
+graph) give byte-identical results to one launch per stack, and the autotuner's tile choice
+never changes results."""
 import numpy as np
 import pytest
 
@@ -103,9 +103,9 @@ def test_batch_against_oracle(ctx):
         assert np.array_equal(m, omask), spec
 
 
-# 8-bit 3-channel candidates per bucket (the packed-byte choices, kernel 1, make batched
-# groups fall back to the bucket's default shape of the batched kernel)
-CANDIDATES_U8 = [[(1, 16, 1), (1, 16, 1, 1)], [(1, 32, 1), (2, 16, 1), (1, 32, 1, 1)],
+# 8-bit 3-channel candidates per bucket: 16-bit-halves shapes (G, F, X), packed-byte byte-search
+# (kernel 1) and sorting-network (kernel 2) shapes; each pins the group kernel the batch uses
+CANDIDATES_U8 = [[(1, 16, 1), (1, 16, 1, 1), (1, 16, 1, 2)], [(1, 32, 1), (2, 16, 1), (1, 32, 1, 1)],
                  [(2, 32, 1), (4, 16, 1), (2, 32, 1, 1)], [(4, 32, 1), (8, 16, 1), (4, 32, 1, 1)],
                  [(8, 32, 1), (4, 64, 1), (8, 32, 1, 1)], [(8, 64, 1), (16, 32, 4, 1), (16, 32, 1, 1)]]
 
@@ -200,19 +200,92 @@ def test_batch_random_mixes(ctx):
 
 @pytest.mark.parametrize("bps", [1, 2])
 def test_autotune_keeps_results(ctx, bps):
-    shapes = cpa.autotune(ctx, bps, 3)
-    assert len(shapes) == len(cpa.TILE_BUCKETS)
-    for sh, nmax in zip(shapes, cpa.TILE_BUCKETS):
+    tuned = cpa.autotune(ctx, bps, 3)
+    assert set(tuned) == {"abs", "rel"}
+    for sh, nmax in zip(tuned["abs"] + tuned["rel"], cpa.TILE_BUCKETS * 2):
         if bps == 2 and nmax > 512:
             assert sh is None
             continue
         g, f, xp, kind = sh
         assert g * f >= nmax and g in (1, 2, 4, 8, 16, 32) and xp in ((1, 4) if bps == 1 else (1, 2))
-        assert kind in ((0, 1) if bps == 1 else (0,))
+        assert kind in ((0, 1, 2) if bps == 1 else (0,))
     for n in (12, 30, 60, 120, 250) + ((700, 1200) if bps == 1 else ()):
         x, st = scene_stack(ctx, 7 + n, n, 70, 24, 3, bps)
-        proc = cpa.OutlierProcessor(cpa.Threshold.abs(0.05, 0.2), 0, 4)
-        img, mask = proc.process(st)
-        oimg, omask, _ = O.outlier(x, O.threshold(True, 0.05, 0.2, bps), 0, 4)
-        assert np.array_equal(img, oimg), n
-        assert np.array_equal(mask, omask), n
+        for absolute, lo, hi, bg, sel in ((True, 0.05, 0.2, 0, 4), (False, 2.0, 4.0, 2, 3)):
+            proc = cpa.OutlierProcessor(cpa.Threshold(absolute, lo, hi), bg, sel)
+            img, mask = proc.process(st)
+            oimg, omask, _ = O.outlier(x, O.threshold(absolute, lo, hi, bps), bg, sel)
+            assert np.array_equal(img, oimg), (n, absolute)
+            assert np.array_equal(mask, omask), (n, absolute)
+
+
+def test_batch_groups_and_side_branches(ctx):
+    """One batch mixing every path: group launches for 8-bit 1/3/4-channel stacks (one with
+    thousands of tiles), a sorting-network choice, `--sample` jobs, tall stacks and 16-bit
+    two-pixel tiles (one with an odd row width, so the staged offsets are realigned row by
+    row); darker and a borrowed stack that ends at its last frame beside them; three graph
+    replays, every job against the oracle."""
+    rng = np.random.default_rng(77)
+    keep, jobs = [], []
+    try:
+        cpa.tile_shapes_set(ctx, 1, 3, [None, (1, 32, 1, 2)], [None, (1, 32, 1, 2)])
+        specs = [(1, 3, 20, 300, 40), (1, 3, 30, 64, 33), (1, 4, 200, 120, 9), (1, 3, 500, 40, 6), (1, 3, 700, 31, 5),
+                 (2, 3, 40, 50, 10), (1, 1, 90, 45, 7), (1, 3, 8, 2000, 400), (2, 3, 100, 51, 12),
+                 (2, 4, 30, 33, 7)]
+        for i, (bps, c, n, w, h) in enumerate(specs):
+            hi = (1 << (8 * bps)) - 1
+            x = rng.integers(hi // 4, 3 * hi // 4, (1, h, w, c)) + rng.integers(-hi // 40, hi // 40 + 1, (n, h, w, c))
+            hit = rng.random((n, h, w)) < 0.1
+            x = np.clip(np.where(hit[..., None], rng.integers(0, hi + 1, (n, h, w, c)), x), 0, hi)
+            x = x.astype(np.uint8 if bps == 1 else np.uint16)
+            st = cpa.Stack.from_array(ctx, x)
+            absolute = i % 2 == 0
+            thr = cpa.Threshold.abs(0.05, 0.2) if absolute else cpa.Threshold.rel(2.0, 4.0)
+            sample = sorted(rng.choice(n, min(n, 50), replace=False).tolist()) if i in (1, 4) else None
+            proc = cpa.OutlierProcessor(thr, i % 4, (i * 5) % 6, sample_indices=sample, seed=3)
+            px, mask = H.DeviceBuffer(x[0].nbytes), H.DeviceBuffer(x[0].size)
+            jobs.append(cpa.Job(cpa.CP_MODE_OUTLIER, st, px.ptr, mask.ptr, processor=proc))
+            keep.append(("outlier", x, st, px, mask, absolute, i % 4, (i * 5) % 6, sample))
+        x = rng.integers(0, 256, (25, 9, 40, 3)).astype(np.uint8)
+        st = cpa.Stack.from_array(ctx, x)
+        px = H.DeviceBuffer(x[0].nbytes)
+        jobs.append(cpa.Job(cpa.CP_MODE_DARKER, st, px.ptr))
+        keep.append(("darker", x, st, px, None, None, None, None, None))
+        # a borrowed stack that ends at its last frame: the wide-load tiles would read past it
+        xw = np.ascontiguousarray(rng.integers(0, 256, (40, 5, 27, 3)).astype(np.uint8))
+        buf = H.DeviceBuffer(xw.nbytes)
+        buf.from_host(xw)
+        stw = cpa.Stack(ctx, 27, 5, 3, 1, device_ptr=buf.ptr, n_frames=40)
+        px = H.DeviceBuffer(xw[0].nbytes)
+        proc = cpa.OutlierProcessor(cpa.Threshold.abs(0.05, 0.2), 0, 2)
+        jobs.append(cpa.Job(cpa.CP_MODE_OUTLIER, stw, px.ptr, None, processor=proc))
+        keep.append(("outlier", xw, stw, px, None, True, 0, 2, None))
+        batch = cpa.Batch(ctx, jobs, graph=True)
+        info = batch.info()
+        # every spec job has a register-resident plan; darker is launched on its own
+        assert info["group_jobs"] >= len(specs) and info["direct_jobs"] >= 1, info
+        assert info["group_jobs"] + info["direct_jobs"] == len(jobs), info
+        for rep in range(3):
+            for k in keep:
+                k[3].from_host(np.zeros(k[3].nbytes, np.uint8))
+            batch.launch()
+            ctx.synchronize()
+            for j, (kind, x, st, px, mask, absolute, bg, sel, sample) in enumerate(keep):
+                img = px.to_host(np.zeros(x[0].shape, x.dtype))
+                if kind == "darker":
+                    assert np.array_equal(img, O.simple(x, darker=True)), (rep, j)
+                    continue
+                thr = O.threshold(absolute, 0.05 if absolute else 2.0, 0.2 if absolute else 4.0, x.dtype.itemsize)
+                oimg, omask, _ = O.outlier(x, thr, bg, sel, seed=3, sample_pos=sample)
+                assert np.array_equal(img, oimg), (rep, j, x.shape)
+                if mask is not None:
+                    assert np.array_equal(mask.to_host(np.zeros(x[0].shape, np.uint8)), omask), (rep, j)
+        batch.close()
+        buf.free()
+    finally:
+        cpa.tile_shapes_set(ctx, 1, 3, [])
+        for k in keep:
+            k[2].close()
+            k[3].free()
+            if k[4] is not None:
+                k[4].free()
