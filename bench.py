@@ -374,7 +374,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="outlier_4k_rgb16_256f", choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-rows", type=int, default=1024, help="output rows of the CPU baseline band")
     ap.add_argument("--no-cpu", action="store_true")
