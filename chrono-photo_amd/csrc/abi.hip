@@ -218,7 +218,9 @@ int cp_ctx::choice_for(uint32_t bps, uint32_t channels, uint32_t n, bool absolut
     const int b = cp::bucket_for(n);
     if (b < 0 || bps < 1 || bps > 2 || channels < 1 || channels > 4) return -1;
     const int t = tuned[bps - 1][channels - 1][absolute ? 0 : 1][b];
-    return t >= 0 ? t : cp::kBucketCand[bps - 1][channels >= 3][b][0];
+    if (t >= 0) return t;
+    const int rel = absolute ? -1 : cp::kRelDefault[bps - 1][channels >= 3][b];
+    return rel >= 0 ? rel : cp::kBucketCand[bps - 1][channels >= 3][b][0];
 }
 
 int cp_ctx::charge(uint64_t bytes) {

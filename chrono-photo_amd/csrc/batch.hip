@@ -456,7 +456,8 @@ extern "C" CP_API int cp_autotune(cp_ctx* ctx, uint32_t bps, uint32_t channels, 
     if (!ctx) return fail(CP_ERR_INVALID_ARG, "null ctx");
     if ((bps != 1 && bps != 2) || channels < 1 || channels > 4)
         return fail(CP_ERR_INVALID_ARG, "autotune covers 8/16-bit stacks with 1 to 4 channels");
-    const uint32_t W = 2048, H = 24, NMAX = kBucketMax[kNumBuckets - 1];
+    // an image of 2048 x 96 pixels: every tile shape fills the device at least 12 times over
+    const uint32_t W = 2048, H = 96, NMAX = kBucketMax[kNumBuckets - 1];
     cp_stack* st = nullptr;
     int rc = cp_stack_create(ctx, W, H, channels, bps, NMAX, &st);
     if (rc) return rc;

@@ -24,8 +24,9 @@ OutlierPlan plan_outlier(cp_ctx* ctx, const cp_stack* st, const OutlierK& k, uin
     // (N * max sample < 2^24: always for 8-bit stacks).  8-bit composites take the packed-byte
     // kernel up to 1280 frames (and, with a `--sample` subset, up to 128 sample frames) or the
     // tuned choice between the two kernels; 16-bit ones the 16-bit kernel up to 512 frames
-    // without a sample subset; the one-lane-per-pixel generic kernel takes the rest, and the
-    // packed-byte composites whose wide loads would read past a borrowed stack's end.
+    // without a sample subset; the one-lane-per-pixel generic kernel takes the rest.  A
+    // packed-byte choice whose wide loads would read past a borrowed stack's end falls back to
+    // the bucket's 16-bit-halves shape, or to the generic kernel where the bucket has none.
     OutlierPlan p;
     const bool sums = k.background == 2 || k.outlier == 3;
     const bool exact_sums = st->bps == 1 || k.n <= 256;
@@ -42,8 +43,14 @@ OutlierPlan plan_outlier(cp_ctx* ctx, const cp_stack* st, const OutlierK& k, uin
     }
     int choice = ctx->choice_for(st->bps, st->channels, k.n, k.th.absolute != 0);
     if (choice >= kU8Base) {
-        if (u8_fits(choice - kU8Base)) p = {kPlanU8, choice - kU8Base};
-    } else if (choice >= 0) {
+        if (u8_fits(choice - kU8Base)) return {kPlanU8, choice - kU8Base};
+        // a borrowed stack the wide loads would overrun: the bucket's 16-bit-halves shape
+        const int b = bucket_for(k.n);
+        choice = b >= 0 && kBucketCand[st->bps - 1][st->channels >= 3][b][0] < kU8Base
+                     ? kBucketCand[st->bps - 1][st->channels >= 3][b][0]
+                     : -1;
+    }
+    if (choice >= 0) {
         if (kShapes[choice].X == 2 && !pair_tile_fits(st->dev, st->alloc_bytes, st->bps, st->channels, k, max_pos))
             choice = narrow_shape(choice);
         if (choice >= 0) p = {kPlanFast, choice};

@@ -32,11 +32,11 @@ constexpr int kMaxCand = 4;
 // [sample bytes - 1][channels >= 3][bucket][candidate]; -1: none
 constexpr int kBucketCand[2][2][kNumBuckets][kMaxCand] = {
     // 8-bit, 1-2 channels: the packed-byte kernel only
-    {{{kU8Base + 0, kU8Base + 14, -1, -1}, {kU8Base + 1, kU8Base + 15, -1, -1}, {kU8Base + 2, -1, -1, -1},
+    {{{kU8Base + 0, kU8Base + 14, -1, -1}, {kU8Base + 1, kU8Base + 15, -1, -1}, {kU8Base + 2, kU8Base + 16, -1, -1},
       {kU8Base + 3, -1, -1, -1}, {kU8Base + 4, -1, -1, -1}, {kU8Base + 6, kU8Base + 5, -1, -1},
       {kU8Base + 7, -1, -1, -1}, {kU8Base + 8, -1, -1, -1}, {kU8Base + 9, -1, -1, -1}},
      // 8-bit, 3-4 channels
-     {{0, kU8Base + 0, kU8Base + 14, -1}, {1, 2, kU8Base + 1, kU8Base + 15}, {3, 4, kU8Base + 2, -1},
+     {{0, kU8Base + 0, kU8Base + 14, -1}, {1, 2, kU8Base + 1, kU8Base + 15}, {3, 4, kU8Base + 2, kU8Base + 16},
       {5, 6, kU8Base + 3, -1}, {7, 8, kU8Base + 4, -1}, {9, kU8Base + 6, kU8Base + 5, -1},
       {kU8Base + 7, -1, -1, -1}, {kU8Base + 8, -1, -1, -1}, {kU8Base + 9, -1, -1, -1}}},
     // 16-bit: 1-2 channels have no register-resident kernel; 3-4 channels up to 512 frames
@@ -44,23 +44,18 @@ constexpr int kBucketCand[2][2][kNumBuckets][kMaxCand] = {
       {-1, -1, -1, -1}, {-1, -1, -1, -1}, {-1, -1, -1, -1}, {-1, -1, -1, -1}},
      {{10, 0, -1, -1}, {11, 1, -1, -1}, {12, 3, -1, -1}, {13, 5, -1, -1}, {14, 7, -1, -1}, {9, -1, -1, -1},
       {-1, -1, -1, -1}, {-1, -1, -1, -1}, {-1, -1, -1, -1}}}};
-// Shapes the batched kernels carry (F <= 32 keeps their register budget).
-constexpr int kBatchShapeMax = 7;
+// Default choice for relative thresholds where it differs from the bucket's first candidate
+// (-1: the first candidate): up to 64 frames of an 8-bit stack, one sort of a lane's bytes
+// gives the median and both quartiles at once, where the byte search runs three searches
+// (64 × 1080p RGB8, rel/2/4: 0.339 ms against 0.428 ms for the best byte-search tile).
+constexpr int kRelDefault[2][2][kNumBuckets] = {
+    {{kU8Base + 14, kU8Base + 15, kU8Base + 16, -1, -1, -1, -1, -1, -1},
+     {kU8Base + 14, kU8Base + 15, kU8Base + 16, -1, -1, -1, -1, -1, -1}},
+    {{-1, -1, -1, -1, -1, -1, -1, -1, -1}, {-1, -1, -1, -1, -1, -1, -1, -1, -1}}};
 
 inline int bucket_for(uint32_t n) {
     for (int b = 0; b < kNumBuckets; ++b)
         if (n <= kBucketMax[b]) return b;
-    return -1;
-}
-
-// The default choice of this kernel's shapes for n frames (batched kernels).
-inline int fast_default_shape(uint32_t bps, uint32_t n) {
-    const int b = bucket_for(n);
-    if (b < 0 || bps < 1 || bps > 2) return -1;
-    for (int c = 0; c < kMaxCand; ++c) {
-        const int ch = kBucketCand[bps - 1][1][b][c];
-        if (ch >= 0 && ch < kU8Base) return ch;
-    }
     return -1;
 }
 

@@ -144,17 +144,17 @@ def test_four_pixel_tiles_with_crops(ctx, n):
         cpa.tile_shapes_set(ctx, 1, 3, [])
 
 
-SORT_CHOICE = [(1, 16, 1, 2), (1, 32, 1, 2)]
+SORT_CHOICE = [(1, 16, 1, 2), (1, 32, 1, 2), (1, 64, 1, 2)]
 
 
 @pytest.mark.parametrize("c", [1, 2, 3, 4])
 def test_sorting_network_ranks(ctx, c):
-    """The one-lane-per-pixel shapes with sorting-network ranks (buckets of up to 16 and 32
-    frames) under every mode family, absolute and relative, with ties, constant pixels and
+    """The one-lane-per-pixel shapes with sorting-network ranks (buckets of up to 16, 32 and
+    64 frames) under every mode family, absolute and relative, with ties, constant pixels and
     pads (frame counts below the network's size)."""
     try:
         cpa.tile_shapes_set(ctx, 1, c, SORT_CHOICE)
-        for n in (1, 2, 3, 5, 16, 17, 31, 32):
+        for n in (1, 2, 3, 5, 16, 17, 31, 32, 33, 50, 64):
             x = make_stack(4500 + n + c, n, 4, 29, c, bits=8, outlier_frac=0.2)
             for bg, sel in MODES[::5] + [(2, 3), (3, 5)]:
                 for absolute, lo, hi in ((True, 0.05, 0.2), (False, 2.0, 4.0)):
