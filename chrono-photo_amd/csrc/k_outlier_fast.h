@@ -141,6 +141,91 @@ __device__ __forceinline__ void byte_select_multi(const uint32_t (&y)[C][ND], co
         }
 }
 
+// 2 #{y < v} - ntot over the group's packed bytes, for v in 1..255.
+template <int G, int ND>
+__device__ __forceinline__ int32_t count_below(const uint32_t (&y)[ND], uint32_t v) {
+    const uint32_t V = v * 0x01010101u, Vm = (v - 1u) * 0x01010101u;
+    uint32_t a = 0, b = 0;
+#pragma unroll
+    for (int i = 0; i < ND; ++i) {
+        a = __builtin_amdgcn_sad_u8(y[i], V, a);
+        b = __builtin_amdgcn_sad_u8(y[i], Vm, b);
+    }
+    return grp_sum<G>((int32_t)(a - b));
+}
+
+// The median and both quartiles of every channel's packed bytes: k = {median rank, first
+// quartile rank <= it, third quartile rank >= it}; out and cnt as byte_select_multi's.  The
+// median comes from the 8-step byte search.  The first quartile then lies in the 16 values
+// [lo, lo + 15] with lo = max(0, median - 15) when #{y < lo} <= its rank (the window's top,
+// median + 1 or above, already counts more than the median's rank), and the third quartile in
+// [lo, lo + 15] with lo = min(median, 240) when #{y < lo + 16} > its rank (#{y < lo} is at
+// most the median's rank): each takes 4 steps instead of 8.  When the two far-end counts
+// show a quartile outside its window for any lane of the wave, the wave runs the full search
+// for both quartiles instead; either way the values are the exact ranks.
+template <int G, int ND, int C>
+__device__ __forceinline__ void byte_select_quartiles(const uint32_t (&y)[C][ND], const uint32_t (&k)[3], int32_t ntot,
+                                                      uint32_t (&out)[C][3], uint32_t (&cnt)[C][3]) {
+    {
+        const uint32_t km[1] = {k[0]};
+        uint32_t o[C][1], n[C][1];
+        byte_select_multi<G, ND, C, 1, true>(y, km, ntot, o, n);
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            out[c][0] = o[c][0];
+            cnt[c][0] = n[c][0];
+        }
+    }
+    const int32_t thrA = 2 * (int32_t)k[1] - ntot, thrB = 2 * (int32_t)k[2] - ntot;
+    uint32_t loA[C], loB[C], cA[C], cB[C];
+    int32_t drA[C], drB[C];
+    bool outside = k[1] > k[0] || k[2] < k[0];  // grid-uniform
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        const uint32_t m = out[c][0];
+        loA[c] = m >= 15u ? m - 15u : 0u;
+        loB[c] = min(m, 240u);
+        const int32_t dA = count_below<G, ND>(y[c], max(loA[c], 1u));
+        const int32_t dB = count_below<G, ND>(y[c], min(loB[c] + 16u, 255u));
+        outside = outside || (loA[c] > 0u && dA > thrA) || (loB[c] < 240u && dB <= thrB);
+        drA[c] = 2 * (int32_t)cnt[c][0] - ntot;       // #{y < median + 1}
+        drB[c] = loB[c] < 240u ? dB : ntot;            // #{y < lo + 16}
+        cA[c] = cB[c] = 0;
+    }
+    if (__any(outside)) {
+        const uint32_t kq[2] = {k[1], k[2]};
+        uint32_t o[C][2], n[C][2];
+        byte_select_multi<G, ND, C, 2, true>(y, kq, ntot, o, n);
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            out[c][1] = o[c][0];
+            cnt[c][1] = n[c][0];
+            out[c][2] = o[c][1];
+            cnt[c][2] = n[c][1];
+        }
+        return;
+    }
+#pragma unroll 1
+    for (int bit = 3; bit >= 0; --bit) {
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            const uint32_t VA = loA[c] + cA[c] + (1u << bit), VB = loB[c] + cB[c] + (1u << bit);
+            const int32_t dA = count_below<G, ND>(y[c], VA), dB = count_below<G, ND>(y[c], VB);
+            if (dA <= thrA) cA[c] += 1u << bit;
+            else drA[c] = dA;
+            if (dB <= thrB) cB[c] += 1u << bit;
+            else drB[c] = dB;
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        out[c][1] = loA[c] + cA[c];
+        cnt[c][1] = (uint32_t)(drA[c] + ntot) >> 1;
+        out[c][2] = loB[c] + cB[c];
+        cnt[c][2] = (uint32_t)(drB[c] + ntot) >> 1;
+    }
+}
+
 // Low bytes of the 16-bit samples (8-bit stacks keep one sample per 16-bit half).
 template <int NW>
 __device__ __forceinline__ void pack_low_bytes(const uint32_t (&r)[NW], uint32_t (&y)[NW / 2]) {
@@ -314,7 +399,7 @@ __device__ __forceinline__ void rank_triples_u8(const uint32_t (&raw)[C][NW], co
     uint32_t y[C][ND], sel[C][3], cle[C][3];
 #pragma unroll
     for (int c = 0; c < C; ++c) pack_low_bytes<NW>(raw[c], y[c]);
-    byte_select_multi<G, ND, C, 3, true>(y, kk, ntot, sel, cle);
+    byte_select_quartiles<G, ND, C>(y, kk, ntot, sel, cle);
 #pragma unroll
     for (int c = 0; c < C; ++c) {
         m0[c] = sel[c][0];

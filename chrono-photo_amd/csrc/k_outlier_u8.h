@@ -163,7 +163,8 @@ __device__ __forceinline__ void ranks_u8(const uint32_t (&y)[C][ND], const RankP
     const uint32_t kk3[3] = {plan.m0, plan.a0, plan.b0};
     const uint32_t(&kk)[R] = *reinterpret_cast<const uint32_t(*)[R]>(&kk3[0]);
     uint32_t sel[C][R], cle[C][R];
-    byte_select_multi<G, ND, C, R, true>(y, kk, ntot, sel, cle);
+    if constexpr (ABS) byte_select_multi<G, ND, C, R, true>(y, kk, ntot, sel, cle);
+    else byte_select_quartiles<G, ND, C>(y, kk3, ntot, sel, cle);
     auto nb = [&](int c, uint32_t v0, uint32_t cnt, uint32_t r0, uint32_t r1) -> uint32_t {
         if (r1 == r0 || cnt >= r1 + 1) return v0;
         return min_greater_u8<G, ND>(y[c], v0);
