@@ -164,3 +164,36 @@ def test_sorting_network_ranks(ctx, c):
                     assert_same(gpu, ref, f"c={c} n={n} bg={bg} sel={sel} abs={absolute}")
     finally:
         cpa.tile_shapes_set(ctx, 1, c, [])
+
+
+# relative-threshold choices per bucket from 64 frames up: the 16-bit-halves byte search and
+# the packed-byte byte search, both of which search the quartiles next to the median
+QUARTILE_CHOICES = [[None, None, (2, 32, 1), (4, 32, 1), (8, 32, 1), (8, 64, 1)],
+                    [None, None, (2, 32, 1, 1), (4, 32, 1, 1), (8, 32, 1, 1), (16, 32, 1, 1)]]
+
+
+@pytest.mark.parametrize("c", [3, 1])
+def test_quartile_windows(ctx, c):
+    """Relative thresholds over pixels whose quartiles lie within 15 levels of the median (the
+    windowed 4-step searches), pixels near 0 and 255 (windows clipped at the range ends),
+    pixels with quartiles far from the median in some lanes of a wave (every wave falls back
+    to the full searches), and rows of each kind alone, for 64-512 frames."""
+    rng = np.random.default_rng(4100 + c)
+    for choice in QUARTILE_CHOICES if c == 3 else QUARTILE_CHOICES[1:]:
+        try:
+            cpa.tile_shapes_set(ctx, 1, c, [], choice)
+            for n in (64, 65, 128, 200, 256, 300, 512):
+                h, w = 6, 70
+                centre = np.array([128, 5, 250, 128, 14, 241])[:, None, None]  # one row per kind
+                x = centre[None] + np.rint(rng.normal(0, 3, (n, h, w, c))).astype(int)
+                wide = rng.integers(0, 256, (n, w, c))
+                x[:, 3] = wide                                  # quartiles far from the median
+                x[:, 5, : w // 2] = wide[:, : w // 2]           # half a row: waves that mix both
+                hit = rng.random((n, h, w)) < 0.05
+                x = np.where(hit[..., None], rng.integers(0, 256, x.shape), x)
+                x = np.clip(x, 0, 255).astype(np.uint8)
+                for bg, sel in ((0, 4), (2, 3), (1, 0)):
+                    gpu, ref = run_both(ctx, x, False, 2.0, 4.0, bg, sel)
+                    assert_same(gpu, ref, f"c={c} n={n} bg={bg} sel={sel} choice={choice}")
+        finally:
+            cpa.tile_shapes_set(ctx, 1, c, [])
