@@ -74,7 +74,11 @@ def build(verbose=False, changed_only=False):
     """Compiles every translation unit (changed_only: only those whose source or local
     headers are newer than their object, for development) and links the libraries."""
     os.makedirs(os.path.join(OUT, "obj"), exist_ok=True)
-    srcs = sorted({s for v in LIBS.values() for s in v})
+    # the kernel translation units take minutes each: start the biggest first (the packed-byte
+    # ones, then the 16-bit-halves ones), so that the small ones fill in at the end
+    def cost(src):
+        return (0 if "_u8_" in src else 1 if "_fast_" in src else 2, src)
+    srcs = sorted({s for v in LIBS.values() for s in v}, key=cost)
     with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
         objs = dict(zip(srcs, ex.map(lambda s: _compile(s, changed_only), srcs)))
     for lib, parts in LIBS.items():

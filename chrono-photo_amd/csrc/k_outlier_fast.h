@@ -968,9 +968,21 @@ __device__ __forceinline__ void outlier_tile(const OutlierK& k, uint32_t bx, uin
                 if constexpr (MAXT) {
                     if (in && d >= lmax[E]) { lmax[E] = d; lmaxi[E] = t0 + (uint32_t)j; }
                 }
-                if constexpr (SUMS) {
+                if constexpr (SUMS && CHECK) {
 #pragma unroll
                     for (int c = 0; c < C; ++c) sum_out[E][c] += (o && in) ? xs[c] : 0u;
+                }
+            }
+            if constexpr (SUMS && !CHECK) {
+                // every frame is real: the outliers' sums from the mask, two frames per
+                // v_dot2_u32_u16 (the mask's two bits spread to one 0/1 half per frame)
+#pragma unroll
+                for (int i = 0; i < NW; ++i) {
+                    const uint32_t b = (m[E][i >> 4] >> (2 * (i & 15))) & 3u;
+                    const uint32_t sel = (b & 1u) | ((b & 2u) << 15);
+#pragma unroll
+                    for (int c = 0; c < C; ++c)
+                        sum_out[E][c] = __builtin_amdgcn_udot2(as_us2(raw[E * C + c][i]), as_us2(sel), sum_out[E][c], false);
                 }
             }
         };
@@ -1009,8 +1021,10 @@ __device__ __forceinline__ void outlier_tile(const OutlierK& k, uint32_t bx, uin
                     else if (pos) classify(e_t, F_{}, T_{}, T_{}, F_{}, F_{});
                     else classify(e_t, F_{}, F_{}, T_{}, F_{}, F_{});
                 } else {
-                    if (want_sums) classify(e_t, F_{}, F_{}, F_{}, T_{}, F_{});
-                    else if (unit) classify(e_t, T_{}, T_{}, F_{}, F_{}, F_{});
+                    if (want_sums) {
+                        if (pos) classify(e_t, F_{}, T_{}, F_{}, T_{}, F_{});
+                        else classify(e_t, F_{}, F_{}, F_{}, T_{}, F_{});
+                    } else if (unit) classify(e_t, T_{}, T_{}, F_{}, F_{}, F_{});
                     else if (pos) classify(e_t, F_{}, T_{}, F_{}, F_{}, F_{});
                     else classify(e_t, F_{}, F_{}, F_{}, F_{}, F_{});
                 }

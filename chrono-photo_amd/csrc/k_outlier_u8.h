@@ -189,10 +189,11 @@ __device__ __forceinline__ void ranks_u8(const uint32_t (&y)[C][ND], const RankP
 // With G = 1 a lane holds every frame of its pixel, so the ranks can come from sorting instead
 // of the byte search: Batcher's odd-even merge sort of the F frames (F a power of two),
 // two channels at a time in the 16-bit halves of a dword (v_pk_min_u16 / v_pk_max_u16 per
-// comparator), then the needed ranks picked from the sorted registers.  Any exact sort gives
-// the reference's values (src/chrono.rs:228-255 sorts too); pads (255) sort to the top.  It
-// does every rank in one pass, which pays off against three byte searches for relative
-// thresholds; the autotuner measures both per frame count.
+// comparator), then the needed ranks picked from the sorted registers; an odd channel (the
+// third of RGB, or the only one) takes the byte search instead.  Any exact sort gives the
+// reference's values (src/chrono.rs:228-255 sorts too); pads (255) sort to the top.  It does
+// every rank of two channels in one pass, which pays off against six byte searches for
+// relative thresholds; the autotuner measures both per frame count.
 struct CxPair {
     int a, b;
 };
@@ -263,10 +264,24 @@ __device__ __forceinline__ void ranks_sort(const uint32_t (&y)[C][F / 4], const 
                                            uint32_t (&m1)[C], uint32_t (&a0)[C], uint32_t (&a1)[C],
                                            uint32_t (&b0)[C], uint32_t (&b1)[C]) {
     static_assert((F & (F - 1)) == 0, "the network sorts a power of two");
-    constexpr int NP = (C + 1) / 2;
+    constexpr int NP = C / 2;  // channel pairs sorted together
+    if constexpr (C % 2 == 1) {
+        // the odd channel alone: a network pass on one channel costs as much as on two, more
+        // than the byte search of one channel's ranks
+        uint32_t yl[1][F / 4], o0[1], o1[1], oa0[1], oa1[1], ob0[1], ob1[1];
+#pragma unroll
+        for (int i = 0; i < F / 4; ++i) yl[0][i] = y[C - 1][i];
+        ranks_u8<1, F / 4, 1, ABS>(yl, plan, F, o0, o1, oa0, oa1, ob0, ob1);
+        m0[C - 1] = o0[0];
+        m1[C - 1] = o1[0];
+        a0[C - 1] = oa0[0];
+        a1[C - 1] = oa1[0];
+        b0[C - 1] = ob0[0];
+        b1[C - 1] = ob1[0];
+    }
 #pragma unroll
     for (int q = 0; q < NP; ++q) {
-        const int c0 = 2 * q, c1 = 2 * q + 1 < C ? 2 * q + 1 : 2 * q;
+        const int c0 = 2 * q, c1 = 2 * q + 1;
         us2 v[F];
 #pragma unroll
         for (int j = 0; j < F; ++j)  // byte j&3 of each channel's dword into the two halves
@@ -276,7 +291,7 @@ __device__ __forceinline__ void ranks_sort(const uint32_t (&y)[C][F / 4], const 
         auto put = [&](uint32_t (&dst)[C], uint32_t r) {
             const us2 s = pick_rank<F>(v, r);
             dst[c0] = s.x;
-            if (c1 != c0) dst[c1] = s.y;
+            dst[c1] = s.y;
         };
         put(m0, plan.m0);
         put(m1, plan.m1);
@@ -410,9 +425,21 @@ __device__ __forceinline__ void outlier_u8_tile(const OutlierK& k, uint32_t bx, 
                         lmaxi[E] = t0 + (uint32_t)j;
                     }
                 }
-                if constexpr (SUMS) {
+                if constexpr (SUMS && CHECK) {
 #pragma unroll
                     for (int c = 0; c < C; ++c) sum_out[E][c] += (o && in) ? xs[c] : 0u;
+                }
+            }
+            if constexpr (SUMS && !CHECK) {
+                // every frame is real: the outliers' sums from the mask, four frames per
+                // v_dot4_u32_u8 (the mask's four bits spread to one 0/1 byte per frame)
+#pragma unroll
+                for (int i = 0; i < ND; ++i) {
+                    const uint32_t nib = (m[E][i >> 3] >> (4 * (i & 7))) & 0xFu;
+                    const uint32_t sel = (nib * 0x00204081u) & 0x01010101u;
+#pragma unroll
+                    for (int c = 0; c < C; ++c)
+                        sum_out[E][c] = __builtin_amdgcn_udot4(yv[E * C + c][i], sel, sum_out[E][c], false);
                 }
             }
         };
@@ -455,8 +482,10 @@ __device__ __forceinline__ void outlier_u8_tile(const OutlierK& k, uint32_t bx, 
                     else if (pos) classify(e_t, F_{}, T_{}, T_{}, F_{}, F_{});
                     else classify(e_t, F_{}, F_{}, T_{}, F_{}, F_{});
                 } else {
-                    if (want_sums) classify(e_t, F_{}, F_{}, F_{}, T_{}, F_{});
-                    else if (unit) classify(e_t, T_{}, T_{}, F_{}, F_{}, F_{});
+                    if (want_sums) {
+                        if (pos) classify(e_t, F_{}, T_{}, F_{}, T_{}, F_{});
+                        else classify(e_t, F_{}, F_{}, F_{}, T_{}, F_{});
+                    } else if (unit) classify(e_t, T_{}, T_{}, F_{}, F_{}, F_{});
                     else if (pos) classify(e_t, F_{}, T_{}, F_{}, F_{}, F_{});
                     else classify(e_t, F_{}, F_{}, F_{}, F_{}, F_{});
                 }
