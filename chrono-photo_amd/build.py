@@ -99,7 +99,7 @@ def build_cli(verbose=False):
     """Host-only C++ (g++): the CLI reaches the GPU through the C ABI alone."""
     out = os.path.join(OUT, "chrono-photo-hip")
     cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-o", out + ".tmp",
-           os.path.join(CLI, "main.cpp"), os.path.join(CLI, "imageio.cpp"),
+           os.path.join(CLI, "main.cpp"), os.path.join(CLI, "imageio.cpp"), os.path.join(CLI, "jpeg.cpp"),
            "-L" + OUT, "-lchrono_hip", "-lz", "-Wl,-rpath,$ORIGIN",
            "-Wl,-rpath-link,/opt/rocm/lib"]
     r = subprocess.run(cmd, capture_output=True, text=True)

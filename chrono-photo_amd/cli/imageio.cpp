@@ -1,5 +1,7 @@
-// imageio.cpp -- PNG and PPM/PGM codecs for the CLI (see imageio.h).
+// imageio.cpp -- PNG and PPM/PGM codecs for the CLI, and the format dispatch (see imageio.h).
 #include "imageio.h"
+
+#include "jpeg.h"
 
 #include <zlib.h>
 
@@ -193,13 +195,19 @@ bool read_image(const std::string& path, Image* img, std::string* err) {
     if (!read_file(path, &f)) { *err = "cannot read " + path; return false; }
     if (f.size() >= 8 && f[0] == 137 && f[1] == 'P') return read_png(f, img, err);
     if (f.size() >= 2 && f[0] == 'P') return read_pnm(f, img, err);
-    *err = path + ": unsupported format (PNG, PPM and PGM are read; JPEG is lossy and not supported)";
+    if (f.size() >= 2 && f[0] == 0xFF && f[1] == 0xD8) {
+        if (read_jpeg(f, img, err)) return true;
+        *err = path + ": " + *err;
+        return false;
+    }
+    *err = path + ": unsupported format (PNG, PPM, PGM and baseline JPEG are read)";
     return false;
 }
 
-bool write_image(const std::string& path, const Image& img, std::string* err) {
+bool write_image(const std::string& path, const Image& img, std::string* err, int quality) {
     if (ends_with(path, ".png")) return write_png(path, img, err);
     if (ends_with(path, ".ppm") || ends_with(path, ".pgm") || ends_with(path, ".pnm")) return write_pnm(path, img, err);
-    *err = path + ": output must be .png, .ppm or .pgm";
+    if (ends_with(path, ".jpg") || ends_with(path, ".jpeg")) return write_jpeg(path, img, quality, false, 0, err);
+    *err = path + ": output must be .png, .ppm, .pgm or .jpg";
     return false;
 }

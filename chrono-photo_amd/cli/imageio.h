@@ -1,5 +1,6 @@
-// imageio.h -- lossless frame I/O for the CLI: PNG (8/16-bit gray, gray+alpha, RGB, RGBA,
-// non-interlaced) through zlib, and binary PPM/PGM (P5/P6, maxval <= 65535).
+// imageio.h -- frame I/O for the CLI: PNG (8/16-bit gray, gray+alpha, RGB, RGBA,
+// non-interlaced) through zlib, binary PPM/PGM (P5/P6, maxval <= 65535), and baseline JPEG
+// (jpeg.h; lossy, 8-bit).
 #pragma once
 #include <cstdint>
 #include <string>
@@ -11,4 +12,5 @@ struct Image {
 };
 
 bool read_image(const std::string& path, Image* img, std::string* err);
-bool write_image(const std::string& path, const Image& img, std::string* err);
+// The format follows the extension; quality applies to .jpg/.jpeg (1-100).
+bool write_image(const std::string& path, const Image& img, std::string* err, int quality = 95);

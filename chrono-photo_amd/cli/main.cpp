@@ -8,7 +8,8 @@
 // then one composite or one per video window (src/main.rs:214-429).  Frames are decoded once
 // into a device-resident stack; the time-slicing flags (--temp-dir, --slice, --compression)
 // and the thread-count flags only tune the reference's disk pipeline and are accepted with a
-// warning.  Images: PNG (8/16-bit) and binary PPM/PGM.  One option is this tool's own:
+// warning.  Images: PNG (8/16-bit), binary PPM/PGM and baseline JPEG (outputs at --quality).
+// One option is this tool's own:
 // --hbm-budget <size>[K|M|G] caps the device memory of the run; a stack larger than that
 // stays in host memory and the composites stream it through the GPU in row bands.
 #include <glob.h>
@@ -79,6 +80,7 @@ struct Args {
     long sample = -1;
     uint64_t seed = 0;
     uint64_t hbm_budget = 0;  // bytes of device memory the run may use (0: no limit)
+    int quality = 95;         // JPEG outputs (src/cli.rs:82-84)
     std::vector<std::string> unused;
 };
 
@@ -147,12 +149,13 @@ Args parse_args(std::vector<std::string> argv) {
             while (i + 1 < argv.size() && argv[i + 1][0] != '-') a.anchors.push_back(argv[++i]);
         } else if (o == "-d" || o == "--debug") a.debug = true;
         else if (o == "-w" || o == "--wait") a.unused.push_back("--wait");
-        else if (o == "--temp-dir" || o == "-c" || o == "--compression" || o == "-s" || o == "--slice" ||
-                 o == "-q" || o == "--quality" || o == "--threads" || o == "--video-threads" ||
-                 o == "--shake-threads") {
+        else if (o == "-q" || o == "--quality") {
             const std::string v = val();
-            if ((o == "-q" || o == "--quality") && (std::stoi(v) < 1 || std::stoi(v) > 100))
-                die("Expected 0 < qualtiy <= 100. Got value " + v);
+            if (std::stoi(v) < 1 || std::stoi(v) > 100) die("Expected 0 < qualtiy <= 100. Got value " + v);
+            a.quality = std::stoi(v);
+        } else if (o == "--temp-dir" || o == "-c" || o == "--compression" || o == "-s" || o == "--slice" ||
+                   o == "--threads" || o == "--video-threads" || o == "--shake-threads") {
+            (void)val();
             a.unused.push_back(o);
         } else die("unknown option " + o);
     }
@@ -358,7 +361,7 @@ int main(int argc, char** argv) {
             else
                 check(cp_outlier(ctx, st, &fr, &pp, &o, 0), "outlier composite");
             if (warnings) printf("Warning: %llu pixels seem to consist of only outliers\n", (unsigned long long)warnings);
-            if (!blend_path.empty() && !write_image(blend_path, mask, &err)) die(err);
+            if (!blend_path.empty() && !write_image(blend_path, mask, &err, a.quality)) die(err);
         } else {
             if (out_of_core)
                 check(cp_simple_host(ctx, host_ptrs.data(), N, W, H, C, B, &fr, a.weights, &fade, a.mode == "darker",
@@ -367,7 +370,7 @@ int main(int argc, char** argv) {
             else
                 check(cp_simple(ctx, st, &fr, a.weights, &fade, a.mode == "darker", img.data.data(), 0), "composite");
         }
-        if (!write_image(out_path, img, &err)) die(err);
+        if (!write_image(out_path, img, &err, a.quality)) die(err);
     };
 
     if (a.has_vin || a.has_vout) {  // create_video, src/main.rs:214-331
