@@ -19,11 +19,12 @@
 // the independent chains interleave.  8-bit stacks need one 8-step level; 16-bit stacks run
 // one level on the high bytes and one on the low bytes clamped into the winning high-byte
 // bucket (values below map to 0, above to 255, which leaves every count #{y < v}, v in
-// [1,255], exact); their medians first try a single level inside a 256-wide window around
-// an estimate and fall back to the two levels when the result lands on the window's edge.  The neighbour order statistic of an even-sized median (and the quartile
-// interpolation partners) is either the same value (when #{x <= v} covers it) or min{x > v}
-// (packed u16 min).  Frames beyond N are padded with the maximum sample value, which never
-// changes the ranks below N.
+// [1,255], exact); their medians first try a single 7-step level inside a 128-wide window
+// around an estimate, then the window next to it, and fall back to the two levels when the
+// result lands on a window's edge.  The neighbour order statistic of an even-sized median
+// (and the quartile interpolation partners) is either the same value (when #{x <= v} covers
+// it) or min{x > v} (packed u16 min).  Frames beyond N are padded with the maximum sample
+// value, which never changes the ranks below N.
 //
 // Classification (src/chrono.rs:257-288) runs over the lane's frames with the reference's
 // f32 operation order; outliers become bits of a per-lane mask.  Each lane then leaves its
@@ -89,16 +90,17 @@ __device__ __forceinline__ uint32_t grp_max_u32(uint32_t v) {
     return v;
 }
 
-// For every channel c and rank k[r]: the largest v in [0, 255] with #{bytes < v} <= k[r]
-// over the group's ND dwords per lane of y[c].  Candidates are kept replicated in all four
-// bytes, so each step is two v_sad_u8 per dword, a G-lane reduction and a select per
-// (channel, rank); the C*R chains are independent and interleave.  With CNT, cnt[c][r]
-// receives #{bytes <= v}: v + 1 is the last rejected candidate (v = prefix 0 1..1 and the
-// step at that 0 bit tested prefix 1 0..0), whose count the search already computed; no
-// rejection means v = 255 and every byte counts.
-template <int G, int ND, int C, int R, bool CNT>
+// For every channel c and rank k[r]: the largest v in [0, 2^BITS - 1] with #{bytes < v} <=
+// k[r] over the group's ND dwords per lane of y[c] (bytes below 2^BITS when BITS < 8).
+// Candidates are kept replicated in all four bytes, so each step is two v_sad_u8 per dword, a
+// G-lane reduction and a select per (channel, rank); the C*R chains are independent and
+// interleave.  With CNT, cnt[c][r] receives #{bytes <= v}: v + 1 is the last rejected
+// candidate (v = prefix 0 1..1 and the step at that 0 bit tested prefix 1 0..0), whose count
+// the search already computed; no rejection means v = 2^BITS - 1 and every byte counts.
+template <int G, int ND, int C, int R, bool CNT, int BITS = 8>
 __device__ __forceinline__ void byte_select_multi(const uint32_t (&y)[C][ND], const uint32_t (&k)[R],
                                                   int32_t ntot, uint32_t (&out)[C][R], uint32_t (&cnt)[C][R]) {
+    static_assert(BITS >= 1 && BITS <= 8, "byte search width");
     uint32_t lo[C][R];
     int32_t drej[C][R], thr[R];
 #pragma unroll
@@ -111,7 +113,7 @@ __device__ __forceinline__ void byte_select_multi(const uint32_t (&y)[C][ND], co
             drej[c][r] = ntot;
         }
 #pragma unroll 1
-    for (int bit = 7; bit >= 0; --bit) {
+    for (int bit = BITS - 1; bit >= 0; --bit) {
         const uint32_t inc = 0x01010101u << bit;
         const uint32_t below = inc - 0x01010101u;  // the lower bits of every byte
 #pragma unroll
@@ -233,15 +235,15 @@ __device__ __forceinline__ void pack_low_bytes(const uint32_t (&r)[NW], uint32_t
     for (int i = 0; i < NW / 2; ++i) y[i] = __builtin_amdgcn_perm(r[2 * i + 1], r[2 * i], 0x06040200u);
 }
 
-// clamp(x - base, 0, 255) of every 16-bit sample, packed four to a dword.
-template <int NW>
+// clamp(x - base, 0, TOP) of every 16-bit sample (TOP <= 255), packed four to a dword.
+template <int NW, uint32_t TOP = 255>
 __device__ __forceinline__ void pack_window(const uint32_t (&r)[NW], uint32_t base, uint32_t (&y)[NW / 2]) {
     const us2 b2 = {(unsigned short)base, (unsigned short)base};
-    const us2 c255 = {255, 255};
+    const us2 ctop = {(unsigned short)TOP, (unsigned short)TOP};
 #pragma unroll
     for (int i = 0; i < NW / 2; ++i) {
-        us2 a = __builtin_elementwise_min(__builtin_elementwise_sub_sat(as_us2(r[2 * i]), b2), c255);
-        us2 b = __builtin_elementwise_min(__builtin_elementwise_sub_sat(as_us2(r[2 * i + 1]), b2), c255);
+        us2 a = __builtin_elementwise_min(__builtin_elementwise_sub_sat(as_us2(r[2 * i]), b2), ctop);
+        us2 b = __builtin_elementwise_min(__builtin_elementwise_sub_sat(as_us2(r[2 * i + 1]), b2), ctop);
         y[i] = __builtin_amdgcn_perm(as_u32(b), as_u32(a), 0x06040200u);
     }
 }
@@ -329,9 +331,11 @@ __device__ __forceinline__ uint32_t neighbour(const uint32_t (&r)[NW], uint32_t 
 
 // Values at ranks (r0, r1), r1 in {r0, r0 + 1}, of every channel.
 //  u8:  one byte level; the neighbour comes from one more byte count.
-//  u16, WINDOW (medians of absolute thresholds): an 8-step byte search inside the window
-//       [lo, lo + 255] around the estimate; a channel's result is exact when it ends strictly
-//       inside (1..254), otherwise that channel runs the exact two-level search.
+//  u16, WINDOW (medians of absolute thresholds): a 7-step byte search inside the window
+//       [lo, lo + 127] around the estimate; a channel's result is exact when it ends strictly
+//       inside (1..126).  Otherwise the rank lies below lo + 1 (result 0) or above lo + 126
+//       (result 127), and that channel searches the window next to it on that side, then, if
+//       that one misses too, runs the exact two-level search.
 //  u16 otherwise: the exact two-level search.
 // Branches depend on group-uniform values only.
 template <int B, int G, int NW, int C, bool WINDOW>
@@ -352,20 +356,29 @@ __device__ __forceinline__ void rank_pairs(const uint32_t (&raw)[C][NW], uint32_
             v1[c] = r1 == r0 ? v0[c] : neighbour<G, NW>(raw[c], v0[c], cle[c][0], r1);
         }
     } else if constexpr (WINDOW) {
+        constexpr uint32_t TOP = 127u, LO_MAX = 65535u - TOP;
         uint32_t y[C][ND], lo[C];
 #pragma unroll
         for (int c = 0; c < C; ++c) {
             const uint32_t est = group_estimate<G, NW>(raw[c]);
-            lo[c] = min(est >= 127u ? est - 127u : 0u, 65535u - 255u);
-            pack_window<NW>(raw[c], lo[c], y[c]);
+            lo[c] = min(est >= 63u ? est - 63u : 0u, LO_MAX);
+            pack_window<NW, TOP>(raw[c], lo[c], y[c]);
         }
-        byte_select_multi<G, ND, C, 1, true>(y, kk, ntot, sel, cle);
+        byte_select_multi<G, ND, C, 1, true, 7>(y, kk, ntot, sel, cle);
 #pragma unroll
         for (int c = 0; c < C; ++c) {
-            const uint32_t ys = sel[c][0];
+            uint32_t ys = sel[c][0], base = lo[c];
             uint32_t le = cle[c][0];  // #{x <= v0} when v0 lies inside the window
-            if (ys >= 1u && ys <= 254u) {
-                v0[c] = lo[c] + ys;
+            if (ys < 1u || ys > TOP - 1u) {
+                base = ys == 0u ? (base >= TOP - 1u ? base - (TOP - 1u) : 0u) : min(base + (TOP - 1u), LO_MAX);
+                uint32_t y1[1][ND], s1[1][1], c1[1][1];
+                pack_window<NW, TOP>(raw[c], base, y1[0]);
+                byte_select_multi<G, ND, 1, 1, true, 7>(y1, kk, ntot, s1, c1);
+                ys = s1[0][0];
+                le = c1[0][0];
+            }
+            if (ys >= 1u && ys <= TOP - 1u) {
+                v0[c] = base + ys;
             } else {
                 uint32_t one[1][NW];
 #pragma unroll

@@ -91,10 +91,10 @@ def test_outlier_u16_wrapped_stack(ctx, c, w, shift):
 @pytest.mark.parametrize("c", [3, 4])
 def test_outlier_u16_full_tiles(ctx, n, c):
     """Frame counts that fill the tile exactly: absolute thresholds take the byte search in a
-    256-wide window around the remedian estimate (exact two-level search where the result
-    lands on the window's edge), relative ones the exact two-level search for the median and
-    both quartiles; noisy pixels, pixels with runs of outliers, constant pixels and pixels at
-    either end of the range all stay exact."""
+    128-wide window around the remedian estimate (the window next to it, then the exact
+    two-level search, where the result lands on a window's edge), relative ones the exact
+    two-level search for the median and both quartiles; noisy pixels, pixels with runs of
+    outliers, constant pixels and pixels at either end of the range all stay exact."""
     for frac, seed in ((0.02, 1), (0.3, 2)):
         x = make_stack(1300 + 10 * n + c + seed, n, 5, 70, c, bits=16, outlier_frac=frac)
         x[:, 1, 2:6, :] = 40000                        # constant pixels
@@ -106,6 +106,56 @@ def test_outlier_u16_full_tiles(ctx, n, c):
             assert_same(gpu, ref, f"n={n} c={c} frac={frac} sel={sel}")
         gpu, ref = run_both(ctx, x, False, 2.0, 4.0, 2, 3)
         assert_same(gpu, ref, f"n={n} c={c} frac={frac} rel")
+
+
+def _window_offsets(x):
+    """Distance of each 16-bit median from the estimate that places its search window, for
+    lanes of 32 frames: per lane the median of three medians of three over the half-words of
+    its first nine frame pairs, averaged over both halves, then the mean over the lanes
+    without the smallest and the largest (the plain mean for two lanes)."""
+    n = x.shape[0]
+    g = n // 32
+    v = np.moveaxis(x.astype(np.int64), 0, -1).reshape(x.shape[1:] + (g, 32))
+    m3 = lambda a, b, c: np.maximum(np.minimum(a, b), np.minimum(np.maximum(a, b), c))
+    e = 0
+    for h in (0, 1):
+        d = [v[..., 2 * i + h] for i in range(9)]
+        e = e + m3(m3(d[0], d[1], d[2]), m3(d[3], d[4], d[5]), m3(d[6], d[7], d[8]))
+    e = e >> 1
+    est = e.sum(-1) >> 1 if g == 2 else (e.sum(-1) - e.min(-1) - e.max(-1)) // (g - 2)
+    med = np.sort(v.reshape(v.shape[:-2] + (n,)), axis=-1)[..., (n + 1) // 2 - 1]
+    return med - est
+
+
+@pytest.mark.parametrize("n", [64, 128, 256])
+def test_outlier_u16_median_windows(ctx, n):
+    """Absolute-threshold medians of 16-bit stacks under every candidate tile shape, with the
+    estimate pushed away from the median pixel by pixel: 12 of each 32-frame lane block (two
+    of each triple of the frame pairs the estimate reads) sit D above or below the rest, so
+    the estimate lands near them while the median stays with the majority.  D from 0 to 945
+    puts the median inside the first 128-wide window, inside the window next to it, and
+    outside both (the exact two-level search); each occurs, and every result is exact."""
+    c, h, w = 3, 2, 136
+    rng = np.random.default_rng(4100 + n)
+    base = rng.integers(20000, 40000, (1, h, w, c))
+    x = base + rng.integers(-4, 5, (n, h, w, c))
+    j = np.arange(n) % 32
+    shifted = np.isin(j, [0, 1, 2, 3, 6, 7, 8, 9, 12, 13, 14, 15])
+    d = (np.arange(w) * 7)[None, None, :, None] * np.array([1, -1, 1])
+    d = np.where(np.arange(h)[None, :, None, None] == 1, d // 2 + 3, d)
+    x = np.where(shifted[:, None, None, None], x + d, x).astype(np.uint16)
+    off = np.abs(_window_offsets(x))
+    if n == 256:
+        assert (off <= 60).any() and ((off >= 70) & (off <= 180)).any() and (off >= 200).any()
+    try:
+        for pick in PICKS:
+            shapes = [cands[min(pick, len(cands) - 1)] for cands in CANDIDATES[16]]
+            cpa.tile_shapes_set(ctx, 2, c, shapes)
+            for sel in (0, 4):
+                gpu, ref = run_both(ctx, x, True, 0.004, 0.02, 0, sel)
+                assert_same(gpu, ref, f"n={n} shape={shapes} sel={sel}")
+    finally:
+        cpa.tile_shapes_set(ctx, 2, c, [])
 
 
 @pytest.mark.parametrize("n", [16, 32, 64, 128, 256, 512])
