@@ -156,26 +156,120 @@ __device__ __forceinline__ int32_t count_below(const uint32_t (&y)[ND], uint32_t
     return grp_sum<G>((int32_t)(a - b));
 }
 
+// us2 median of three (per 16-bit half).
+__device__ __forceinline__ us2 med3_us2(us2 a, us2 b, us2 c) {
+    const us2 lo = __builtin_elementwise_min(a, b), hi = __builtin_elementwise_max(a, b);
+    return __builtin_elementwise_max(lo, __builtin_elementwise_min(hi, c));
+}
+
+// Location estimate of one channel's packed bytes over the group, which only places a search
+// window (exactness never depends on it).  A lane with its first 12 frames real (ok) offers
+// the mean of four medians of three of them; the group takes the mean of the offers without
+// the smallest and the largest (all of them with fewer than three), nv = the number of
+// offering lanes (nonzero).
+template <int G, int ND>
+__device__ __forceinline__ uint32_t byte_estimate(const uint32_t (&y)[ND], bool ok, uint32_t nv) {
+    static_assert(ND >= 3, "estimate reads three dwords");
+    us2 a[6];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {  // frames 4i, 4i+2 and 4i+1, 4i+3 widened to 16-bit halves
+        a[2 * i] = as_us2(__builtin_amdgcn_perm(0u, y[i], 0x0C020C00u));
+        a[2 * i + 1] = as_us2(__builtin_amdgcn_perm(0u, y[i], 0x0C030C01u));
+    }
+    const us2 m = med3_us2(a[0], a[1], a[2]), n = med3_us2(a[3], a[4], a[5]);
+    const uint32_t e = ((uint32_t)m.x + (uint32_t)m.y + (uint32_t)n.x + (uint32_t)n.y) >> 2;
+    if constexpr (G == 1) return e;
+    const uint32_t s = (uint32_t)grp_sum<G>((int32_t)(ok ? e : 0u));
+    uint32_t num = s, den = nv;
+    if constexpr (G >= 4) {
+        if (nv >= 3u) {
+            num = s - grp_min_u32<G>(ok ? e : 0xFFFFu) - grp_max_u32<G>(ok ? e : 0u);
+            den = nv - 2u;
+        }
+    }
+    return den == 1u ? num : (uint32_t)((float)num * __builtin_amdgcn_rcpf((float)den));
+}
+
+// The k-th smallest of every channel's packed bytes (out) and #{bytes <= it} (cnt), as
+// byte_select_multi's single rank: a 4-step search in the 16 values [lo, lo + 15] around the
+// estimate, lo = clamp(estimate - 7, 0, 240).  The result v = lo + j is exact when 1 <= j <= 14:
+// #{y < v} <= k was counted when j's last 1 bit was set and #{y < v + 1} > k when that
+// candidate was rejected; j = 0 at lo = 0 and j = 15 at lo = 240 are exact too (the value is 0
+// or 255).  A channel whose result lands on another edge runs the 8-step search; so does every
+// channel when no lane of the group has 12 real frames (real: the lane's frames below the
+// stack's count).
+template <int G, int ND, int C>
+__device__ __forceinline__ void byte_select_median(const uint32_t (&y)[C][ND], uint32_t k, int32_t ntot,
+                                                   uint32_t real, uint32_t (&out)[C], uint32_t (&cnt)[C]) {
+    const uint32_t kk[1] = {k};
+    const bool ok = real >= 12u;
+    const uint32_t nv = G == 1 ? (ok ? 1u : 0u) : (uint32_t)grp_sum<G>(ok ? 1 : 0);
+    if (ND < 3 || nv == 0u) {  // uniform over the wave: the same frame count in every group
+        uint32_t o[C][1], n[C][1];
+        byte_select_multi<G, ND, C, 1, true>(y, kk, ntot, o, n);
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            out[c] = o[c][0];
+            cnt[c] = n[c][0];
+        }
+        return;
+    }
+    if constexpr (ND >= 3) {
+        const int32_t thr = 2 * (int32_t)k - ntot;
+        uint32_t lo[C], j[C];
+        int32_t drej[C];
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            const uint32_t est = byte_estimate<G, ND>(y[c], ok, nv);
+            lo[c] = min(est >= 7u ? est - 7u : 0u, 240u);
+            j[c] = 0;
+            drej[c] = ntot;
+        }
+#pragma unroll 1
+        for (int bit = 3; bit >= 0; --bit) {
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                const int32_t d = count_below<G, ND>(y[c], lo[c] + j[c] + (1u << bit));
+                if (d <= thr) j[c] += 1u << bit;
+                else drej[c] = d;
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            if ((j[c] >= 1u || lo[c] == 0u) && (j[c] <= 14u || lo[c] == 240u)) {
+                out[c] = lo[c] + j[c];
+                cnt[c] = (uint32_t)(drej[c] + ntot) >> 1;
+            } else {
+                uint32_t one[1][ND], o[1][1], n[1][1];
+#pragma unroll
+                for (int i = 0; i < ND; ++i) one[0][i] = y[c][i];
+                byte_select_multi<G, ND, 1, 1, true>(one, kk, ntot, o, n);
+                out[c] = o[0][0];
+                cnt[c] = n[0][0];
+            }
+        }
+    }
+}
+
 // The median and both quartiles of every channel's packed bytes: k = {median rank, first
 // quartile rank <= it, third quartile rank >= it}; out and cnt as byte_select_multi's.  The
-// median comes from the 8-step byte search.  The first quartile then lies in the 16 values
-// [lo, lo + 15] with lo = max(0, median - 15) when #{y < lo} <= its rank (the window's top,
-// median + 1 or above, already counts more than the median's rank), and the third quartile in
-// [lo, lo + 15] with lo = min(median, 240) when #{y < lo + 16} > its rank (#{y < lo} is at
-// most the median's rank): each takes 4 steps instead of 8.  When the two far-end counts
+// median comes from byte_select_median (real as there).  The first quartile then lies in the
+// 16 values [lo, lo + 15] with lo = max(0, median - 15) when #{y < lo} <= its rank (the
+// window's top, median + 1 or above, already counts more than the median's rank), and the
+// third quartile in [lo, lo + 15] with lo = min(median, 240) when #{y < lo + 16} > its rank
+// (#{y < lo} is at most the median's rank): each takes 4 steps instead of 8.  When the two far-end counts
 // show a quartile outside its window for any lane of the wave, the wave runs the full search
 // for both quartiles instead; either way the values are the exact ranks.
 template <int G, int ND, int C>
 __device__ __forceinline__ void byte_select_quartiles(const uint32_t (&y)[C][ND], const uint32_t (&k)[3], int32_t ntot,
-                                                      uint32_t (&out)[C][3], uint32_t (&cnt)[C][3]) {
+                                                      uint32_t real, uint32_t (&out)[C][3], uint32_t (&cnt)[C][3]) {
     {
-        const uint32_t km[1] = {k[0]};
-        uint32_t o[C][1], n[C][1];
-        byte_select_multi<G, ND, C, 1, true>(y, km, ntot, o, n);
+        uint32_t o[C], n[C];
+        byte_select_median<G, ND, C>(y, k[0], ntot, real, o, n);
 #pragma unroll
         for (int c = 0; c < C; ++c) {
-            out[c][0] = o[c][0];
-            cnt[c][0] = n[c][0];
+            out[c][0] = o[c];
+            cnt[c][0] = n[c];
         }
     }
     const int32_t thrA = 2 * (int32_t)k[1] - ntot, thrB = 2 * (int32_t)k[2] - ntot;
@@ -293,11 +387,6 @@ __device__ __forceinline__ uint32_t min_greater(const uint32_t (&r)[NW], uint32_
     return grp_min_u32<G>(mm) + v + 1;
 }
 
-__device__ __forceinline__ us2 med3_us2(us2 a, us2 b, us2 c) {
-    const us2 lo = __builtin_elementwise_min(a, b), hi = __builtin_elementwise_max(a, b);
-    return __builtin_elementwise_max(lo, __builtin_elementwise_min(hi, c));
-}
-
 // Location estimate of a 16-bit channel: median-of-3 "remedian" over the lane's samples
 // (both halves of each dword at once), then a trimmed mean over the group's lanes.  It only
 // places the search window; exactness never depends on it.
@@ -330,7 +419,8 @@ __device__ __forceinline__ uint32_t neighbour(const uint32_t (&r)[NW], uint32_t 
 }
 
 // Values at ranks (r0, r1), r1 in {r0, r0 + 1}, of every channel.
-//  u8:  one byte level; the neighbour comes from one more byte count.
+//  u8:  one byte level (WINDOW: in the 16-value window of byte_select_median, real = the lane
+//       block's frames below the stack's count); the neighbour is v0 or min{x > v0}.
 //  u16, WINDOW (medians of absolute thresholds): a 7-step byte search inside the window
 //       [lo, lo + 127] around the estimate; a channel's result is exact when it ends strictly
 //       inside (1..126).  Otherwise the rank lies below lo + 1 (result 0) or above lo + 126
@@ -340,7 +430,7 @@ __device__ __forceinline__ uint32_t neighbour(const uint32_t (&r)[NW], uint32_t 
 // Branches depend on group-uniform values only.
 template <int B, int G, int NW, int C, bool WINDOW>
 __device__ __forceinline__ void rank_pairs(const uint32_t (&raw)[C][NW], uint32_t r0, uint32_t r1, int32_t ntot,
-                                           uint32_t (&v0)[C], uint32_t (&v1)[C]) {
+                                           uint32_t real, uint32_t (&v0)[C], uint32_t (&v1)[C]) {
     constexpr int ND = NW / 2;
     constexpr uint32_t MAXV = B == 1 ? 255u : 65535u;
     const uint32_t kk[1] = {r0};
@@ -349,11 +439,21 @@ __device__ __forceinline__ void rank_pairs(const uint32_t (&raw)[C][NW], uint32_
         uint32_t y[C][ND];
 #pragma unroll
         for (int c = 0; c < C; ++c) pack_low_bytes<NW>(raw[c], y[c]);
-        byte_select_multi<G, ND, C, 1, true>(y, kk, ntot, sel, cle);
+        uint32_t s1[C], n1[C];
+        if constexpr (WINDOW) {
+            byte_select_median<G, ND, C>(y, r0, ntot, real, s1, n1);
+        } else {
+            byte_select_multi<G, ND, C, 1, true>(y, kk, ntot, sel, cle);
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                s1[c] = sel[c][0];
+                n1[c] = cle[c][0];
+            }
+        }
 #pragma unroll
         for (int c = 0; c < C; ++c) {
-            v0[c] = sel[c][0];
-            v1[c] = r1 == r0 ? v0[c] : neighbour<G, NW>(raw[c], v0[c], cle[c][0], r1);
+            v0[c] = s1[c];
+            v1[c] = r1 == r0 ? v0[c] : neighbour<G, NW>(raw[c], v0[c], n1[c], r1);
         }
     } else if constexpr (WINDOW) {
         constexpr uint32_t TOP = 127u, LO_MAX = 65535u - TOP;
@@ -405,14 +505,14 @@ __device__ __forceinline__ void rank_pairs(const uint32_t (&raw)[C][NW], uint32_
 // 8-bit stack in one byte search: the bytes are packed once and the 3*C chains interleave.
 template <int G, int NW, int C>
 __device__ __forceinline__ void rank_triples_u8(const uint32_t (&raw)[C][NW], const RankPlan& plan, int32_t ntot,
-                                                uint32_t (&m0)[C], uint32_t (&m1)[C], uint32_t (&a0)[C],
+                                                uint32_t real, uint32_t (&m0)[C], uint32_t (&m1)[C], uint32_t (&a0)[C],
                                                 uint32_t (&a1)[C], uint32_t (&b0)[C], uint32_t (&b1)[C]) {
     constexpr int ND = NW / 2;
     const uint32_t kk[3] = {plan.m0, plan.a0, plan.b0};
     uint32_t y[C][ND], sel[C][3], cle[C][3];
 #pragma unroll
     for (int c = 0; c < C; ++c) pack_low_bytes<NW>(raw[c], y[c]);
-    byte_select_quartiles<G, ND, C>(y, kk, ntot, sel, cle);
+    byte_select_quartiles<G, ND, C>(y, kk, ntot, real, sel, cle);
 #pragma unroll
     for (int c = 0; c < C; ++c) {
         m0[c] = sel[c][0];
@@ -891,18 +991,19 @@ __device__ __forceinline__ void outlier_tile(const OutlierK& k, uint32_t bx, uin
 
         // ---- medians / inverse IQR per pixel and channel
         const RankPlan plan = rank_plan_core(N, ABS);
+        const uint32_t real = N > t0 ? min(N - t0, (uint32_t)F) : 0u;  // real frames in this block
         float med[X][4], iqr_inv[X][4];
         StaticFor<0, X>::run([&](auto e_t) {
             constexpr int E = decltype(e_t)::value;
             const uint32_t (&re)[C][NW] = *reinterpret_cast<const uint32_t(*)[C][NW]>(&raw[E * C][0]);
             uint32_t m0[C], m1[C], a0[C], a1[C], b0[C], b1[C];
             if constexpr (!ABS && B == 1) {
-                rank_triples_u8<G, NW, C>(re, plan, ntot, m0, m1, a0, a1, b0, b1);
+                rank_triples_u8<G, NW, C>(re, plan, ntot, real, m0, m1, a0, a1, b0, b1);
             } else {
-                rank_pairs<B, G, NW, C, ABS>(re, plan.m0, plan.m1, ntot, m0, m1);
+                rank_pairs<B, G, NW, C, ABS>(re, plan.m0, plan.m1, ntot, real, m0, m1);
                 if constexpr (!ABS) {
-                    rank_pairs<B, G, NW, C, false>(re, plan.a0, plan.a1, ntot, a0, a1);
-                    rank_pairs<B, G, NW, C, false>(re, plan.b0, plan.b1, ntot, b0, b1);
+                    rank_pairs<B, G, NW, C, false>(re, plan.a0, plan.a1, ntot, real, a0, a1);
+                    rank_pairs<B, G, NW, C, false>(re, plan.b0, plan.b1, ntot, real, b0, b1);
                 }
             }
 #pragma unroll

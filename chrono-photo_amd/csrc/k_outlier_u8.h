@@ -154,17 +154,27 @@ __device__ __forceinline__ uint32_t min_greater_u8(const uint32_t (&y)[ND], uint
 }
 
 // Values at ranks (r0, r1), r1 in {r0, r0 + 1}, for the median and, for relative thresholds,
-// both quartiles of every channel, in one byte search whose C * R chains interleave.
+// both quartiles of every channel: the median from a 16-value window around an estimate
+// (byte_select_median; real = the lane block's frames below the count the ranks are over), the
+// quartiles from windows next to it, the C chains of each search interleaved.
 template <int G, int ND, int C, bool ABS>
 __device__ __forceinline__ void ranks_u8(const uint32_t (&y)[C][ND], const RankPlan& plan, int32_t ntot,
-                                         uint32_t (&m0)[C], uint32_t (&m1)[C], uint32_t (&a0)[C],
+                                         uint32_t real, uint32_t (&m0)[C], uint32_t (&m1)[C], uint32_t (&a0)[C],
                                          uint32_t (&a1)[C], uint32_t (&b0)[C], uint32_t (&b1)[C]) {
     constexpr int R = ABS ? 1 : 3;
     const uint32_t kk3[3] = {plan.m0, plan.a0, plan.b0};
-    const uint32_t(&kk)[R] = *reinterpret_cast<const uint32_t(*)[R]>(&kk3[0]);
     uint32_t sel[C][R], cle[C][R];
-    if constexpr (ABS) byte_select_multi<G, ND, C, R, true>(y, kk, ntot, sel, cle);
-    else byte_select_quartiles<G, ND, C>(y, kk3, ntot, sel, cle);
+    if constexpr (ABS) {
+        uint32_t o[C], n[C];
+        byte_select_median<G, ND, C>(y, plan.m0, ntot, real, o, n);
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            sel[c][0] = o[c];
+            cle[c][0] = n[c];
+        }
+    } else {
+        byte_select_quartiles<G, ND, C>(y, kk3, ntot, real, sel, cle);
+    }
     auto nb = [&](int c, uint32_t v0, uint32_t cnt, uint32_t r0, uint32_t r1) -> uint32_t {
         if (r1 == r0 || cnt >= r1 + 1) return v0;
         return min_greater_u8<G, ND>(y[c], v0);
@@ -260,7 +270,8 @@ __device__ __forceinline__ us2 pick_rank(const us2 (&v)[N], uint32_t r) {
 
 // The rank values of ranks_u8 from a sort of the lane's F frames (G = 1).
 template <int F, int C, bool ABS>
-__device__ __forceinline__ void ranks_sort(const uint32_t (&y)[C][F / 4], const RankPlan& plan, uint32_t (&m0)[C],
+__device__ __forceinline__ void ranks_sort(const uint32_t (&y)[C][F / 4], const RankPlan& plan, uint32_t real,
+                                           uint32_t (&m0)[C],
                                            uint32_t (&m1)[C], uint32_t (&a0)[C], uint32_t (&a1)[C],
                                            uint32_t (&b0)[C], uint32_t (&b1)[C]) {
     static_assert((F & (F - 1)) == 0, "the network sorts a power of two");
@@ -271,7 +282,7 @@ __device__ __forceinline__ void ranks_sort(const uint32_t (&y)[C][F / 4], const 
         uint32_t yl[1][F / 4], o0[1], o1[1], oa0[1], oa1[1], ob0[1], ob1[1];
 #pragma unroll
         for (int i = 0; i < F / 4; ++i) yl[0][i] = y[C - 1][i];
-        ranks_u8<1, F / 4, 1, ABS>(yl, plan, F, o0, o1, oa0, oa1, ob0, ob1);
+        ranks_u8<1, F / 4, 1, ABS>(yl, plan, F, real, o0, o1, oa0, oa1, ob0, ob1);
         m0[C - 1] = o0[0];
         m1[C - 1] = o1[0];
         a0[C - 1] = oa0[0];
@@ -338,15 +349,18 @@ __device__ __forceinline__ void outlier_u8_tile(const OutlierK& k, uint32_t bx, 
         const RankPlan plan = rank_plan_core(k.n_sample, ABS);
         uint32_t m0[XC], m1[XC], a0[XC], a1[XC], b0[XC], b1[XC];
         if constexpr (SORT) {
-            ranks_sort<F, XC, ABS>(yv, plan, m0, m1, a0, a1, b0, b1);
+            ranks_sort<F, XC, ABS>(yv, plan, min(N, (uint32_t)F), m0, m1, a0, a1, b0, b1);
         } else if constexpr (FS > 0) {
             constexpr int NDS = FS / 4;
             uint32_t ys[XC][NDS];
             load_block_u8<C, X, FS>(base, soff + g * (FS + 2), ys, g * FS, k.n_sample,
                                     k.n_sample < (uint32_t)(G * FS));
-            ranks_u8<G, NDS, XC, ABS>(ys, plan, G * FS, m0, m1, a0, a1, b0, b1);
+            const uint32_t ts = g * FS, ns = k.n_sample;
+            ranks_u8<G, NDS, XC, ABS>(ys, plan, G * FS, ns > ts ? min(ns - ts, (uint32_t)FS) : 0u, m0, m1, a0, a1,
+                                      b0, b1);
         } else {
-            ranks_u8<G, ND, XC, ABS>(yv, plan, G * F, m0, m1, a0, a1, b0, b1);
+            ranks_u8<G, ND, XC, ABS>(yv, plan, G * F, N > t0 ? min(N - t0, (uint32_t)F) : 0u, m0, m1, a0, a1, b0,
+                                     b1);
         }
         float med[X][4], iqr_inv[X][4];
 #pragma unroll

@@ -197,3 +197,35 @@ def test_quartile_windows(ctx, c):
                     assert_same(gpu, ref, f"c={c} n={n} bg={bg} sel={sel} choice={choice}")
         finally:
             cpa.tile_shapes_set(ctx, 1, c, [])
+
+
+@pytest.mark.parametrize("c", [3, 1])
+def test_median_windows(ctx, c):
+    """8-bit medians searched in a 16-value window around an estimate, under every candidate
+    tile shape of every bucket: in each group of 8 frames the first 4 sit D above the rest
+    (two of each triple of frames the estimate reads), so the estimate lands about D above a
+    median that stays with the lower half.  D from 0 to 29 per column puts the median inside
+    the window or past its edge (the 8-step search); rows near 0 and 255 clip the window at
+    the range ends; stacks below 12 frames per lane block take the 8-step search directly."""
+    from test_parity_gpu import CANDIDATES, PICKS
+    rng = np.random.default_rng(4300 + c)
+    h, w = 3, 60
+    if c == 3:
+        sets = [[cands[min(pick, len(cands) - 1)] for cands in CANDIDATES[8]] for pick in PICKS]
+    else:
+        sets = U8_CHOICES + [SORT_CHOICE]
+    for shapes in sets:
+        try:
+            cpa.tile_shapes_set(ctx, 1, c, shapes)
+            for n in (8, 16, 33, 64, 100, 256, 300, 512, 640, 1280):
+                base = np.array([120, 2, 248])[:, None, None]
+                d = (np.arange(w) % 30)[None, :, None]
+                x = base[None] + rng.integers(-1, 2, (n, h, w, c))
+                shifted = (np.arange(n) % 8) < 4
+                x = np.where(shifted[:, None, None, None], x + d[None], x)
+                x = np.clip(x, 0, 255).astype(np.uint8)
+                for absolute, lo, hi, bg, sel in ((True, 0.05, 0.2, 0, 4), (False, 2.0, 4.0, 2, 3)):
+                    gpu, ref = run_both(ctx, x, absolute, lo, hi, bg, sel)
+                    assert_same(gpu, ref, f"c={c} n={n} shapes={shapes} abs={absolute}")
+        finally:
+            cpa.tile_shapes_set(ctx, 1, c, [])
