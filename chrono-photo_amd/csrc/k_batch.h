@@ -118,6 +118,7 @@ const void* u8_group_c(int shape, bool absolute) {
     case 14: return u8_group<C, 1, 16, 1, 0, true>(absolute);
     case 15: return u8_group<C, 1, 32, 1, 0, true>(absolute);
     case 16: return u8_group<C, 1, 64, 1, 0, true>(absolute);
+    case 17: return u8_group<C, 1, 64, 1, 0>(absolute);
     default: return nullptr;
     }
 }

@@ -28,22 +28,25 @@ constexpr TileShape kShapes[kNumShapes] = {{1, 16, 1}, {1, 32, 1}, {2, 16, 1}, {
 constexpr int kNumBuckets = 9;
 constexpr uint32_t kBucketMax[kNumBuckets] = {16, 32, 64, 128, 256, 512, 768, 1024, 1280};
 constexpr int kU8Base = 64;
-constexpr int kMaxCand = 4;
+constexpr int kMaxCand = 5;
 // [sample bytes - 1][channels >= 3][bucket][candidate]; -1: none
 constexpr int kBucketCand[2][2][kNumBuckets][kMaxCand] = {
     // 8-bit, 1-2 channels: the packed-byte kernel only
-    {{{kU8Base + 0, kU8Base + 14, -1, -1}, {kU8Base + 1, kU8Base + 15, -1, -1}, {kU8Base + 2, kU8Base + 16, -1, -1},
-      {kU8Base + 3, -1, -1, -1}, {kU8Base + 4, -1, -1, -1}, {kU8Base + 6, kU8Base + 5, -1, -1},
-      {kU8Base + 7, -1, -1, -1}, {kU8Base + 8, -1, -1, -1}, {kU8Base + 9, -1, -1, -1}},
+    {{{kU8Base + 0, kU8Base + 14, -1, -1, -1}, {kU8Base + 1, kU8Base + 15, -1, -1, -1},
+      {kU8Base + 2, kU8Base + 16, kU8Base + 17, -1, -1}, {kU8Base + 3, -1, -1, -1, -1},
+      {kU8Base + 4, -1, -1, -1, -1}, {kU8Base + 6, kU8Base + 5, -1, -1, -1}, {kU8Base + 7, -1, -1, -1, -1},
+      {kU8Base + 8, -1, -1, -1, -1}, {kU8Base + 9, -1, -1, -1, -1}},
      // 8-bit, 3-4 channels
-     {{0, kU8Base + 0, kU8Base + 14, -1}, {1, 2, kU8Base + 1, kU8Base + 15}, {3, 4, kU8Base + 2, kU8Base + 16},
-      {5, 6, kU8Base + 3, -1}, {7, 8, kU8Base + 4, -1}, {9, kU8Base + 6, kU8Base + 5, -1},
-      {kU8Base + 7, -1, -1, -1}, {kU8Base + 8, -1, -1, -1}, {kU8Base + 9, -1, -1, -1}}},
+     {{0, kU8Base + 0, kU8Base + 14, -1, -1}, {1, 2, kU8Base + 1, kU8Base + 15, -1},
+      {3, 4, kU8Base + 2, kU8Base + 16, kU8Base + 17}, {5, 6, kU8Base + 3, -1, -1},
+      {7, 8, kU8Base + 4, -1, -1}, {9, kU8Base + 6, kU8Base + 5, -1, -1}, {kU8Base + 7, -1, -1, -1, -1},
+      {kU8Base + 8, -1, -1, -1, -1}, {kU8Base + 9, -1, -1, -1, -1}}},
     // 16-bit: 1-2 channels have no register-resident kernel; 3-4 channels up to 512 frames
-    {{{-1, -1, -1, -1}, {-1, -1, -1, -1}, {-1, -1, -1, -1}, {-1, -1, -1, -1}, {-1, -1, -1, -1},
-      {-1, -1, -1, -1}, {-1, -1, -1, -1}, {-1, -1, -1, -1}, {-1, -1, -1, -1}},
-     {{10, 0, -1, -1}, {11, 1, -1, -1}, {12, 3, -1, -1}, {13, 5, -1, -1}, {14, 7, -1, -1}, {9, -1, -1, -1},
-      {-1, -1, -1, -1}, {-1, -1, -1, -1}, {-1, -1, -1, -1}}}};
+    {{{-1, -1, -1, -1, -1}, {-1, -1, -1, -1, -1}, {-1, -1, -1, -1, -1}, {-1, -1, -1, -1, -1},
+      {-1, -1, -1, -1, -1}, {-1, -1, -1, -1, -1}, {-1, -1, -1, -1, -1}, {-1, -1, -1, -1, -1},
+      {-1, -1, -1, -1, -1}},
+     {{10, 0, -1, -1, -1}, {11, 1, -1, -1, -1}, {12, 3, -1, -1, -1}, {13, 5, -1, -1, -1}, {14, 7, -1, -1, -1},
+      {9, -1, -1, -1, -1}, {-1, -1, -1, -1, -1}, {-1, -1, -1, -1, -1}, {-1, -1, -1, -1, -1}}}};
 // Default choice for relative thresholds where it differs from the bucket's first candidate
 // (-1: the first candidate): up to 64 frames of an 8-bit stack, one sort of a lane's bytes
 // gives the median and both quartiles at once, where the byte search runs three searches

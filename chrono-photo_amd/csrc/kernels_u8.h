@@ -13,14 +13,15 @@ struct U8Shape {
 };
 
 // One pixel per lane for short stacks; four pixels per lane (one 16-byte load per frame)
-// for tall ones; `--sample` shapes carry an FS-frame block of the sample positions; the
-// last three rank one lane's whole pixel with a sorting network.
-constexpr int kNumU8Shapes = 17;
+// for tall ones; `--sample` shapes carry an FS-frame block of the sample positions; shapes
+// 14-16 rank one lane's whole pixel with a sorting network; shape 17 holds a 64-frame pixel
+// in one lane for the byte search (a candidate only: u8_shape_for meets shape 2 first).
+constexpr int kNumU8Shapes = 18;
 constexpr U8Shape kU8Shapes[kNumU8Shapes] = {
     {1, 16, 1, 0, 0},  {1, 32, 1, 0, 0},  {2, 32, 1, 0, 0},  {4, 32, 1, 0, 0}, {8, 32, 1, 0, 0},
     {16, 32, 1, 0, 0}, {16, 32, 4, 0, 0}, {32, 24, 4, 0, 0}, {32, 32, 4, 0, 0}, {32, 40, 4, 0, 0},
     {4, 32, 1, 8, 0},  {8, 32, 1, 8, 0},  {16, 32, 4, 8, 0}, {32, 40, 4, 4, 0}, {1, 16, 1, 0, 1},
-    {1, 32, 1, 0, 1},  {1, 64, 1, 0, 1}};
+    {1, 32, 1, 0, 1},  {1, 64, 1, 0, 1},  {1, 64, 1, 0, 0}};
 constexpr uint32_t kU8MaxFrames = 32 * 40;
 constexpr uint32_t kU8MaxSample = 128;
 
@@ -65,6 +66,7 @@ inline bool launch_u8_c(const OutlierK& k, int shape, hipStream_t s) {
     case 14: launch_u8_shape<C, 1, 16, 1, 0, true>(k, s); return true;
     case 15: launch_u8_shape<C, 1, 32, 1, 0, true>(k, s); return true;
     case 16: launch_u8_shape<C, 1, 64, 1, 0, true>(k, s); return true;
+    case 17: launch_u8_shape<C, 1, 64, 1, 0>(k, s); return true;
     default: return false;
     }
 }

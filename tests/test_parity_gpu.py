@@ -372,13 +372,14 @@ def test_outlier_frame_counts(ctx, n):
 # 256, 512, 768, 1024, 1280 frames
 CANDIDATES = {
     8: [[(1, 16, 1), (1, 16, 1, 1), (1, 16, 1, 2)], [(1, 32, 1), (2, 16, 1), (1, 32, 1, 1), (1, 32, 1, 2)],
-        [(2, 32, 1), (4, 16, 1), (2, 32, 1, 1), (1, 64, 1, 2)], [(4, 32, 1), (8, 16, 1), (4, 32, 1, 1)],
-        [(8, 32, 1), (4, 64, 1), (8, 32, 1, 1)], [(8, 64, 1), (16, 32, 4, 1), (16, 32, 1, 1)],
+        [(2, 32, 1), (4, 16, 1), (2, 32, 1, 1), (1, 64, 1, 2), (1, 64, 1, 1)],
+        [(4, 32, 1), (8, 16, 1), (4, 32, 1, 1)], [(8, 32, 1), (4, 64, 1), (8, 32, 1, 1)],
+        [(8, 64, 1), (16, 32, 4, 1), (16, 32, 1, 1)],
         [(32, 24, 4, 1)], [(32, 32, 4, 1)], [(32, 40, 4, 1)]],
     16: [[(1, 16, 2), (1, 16, 1)], [(1, 32, 2), (1, 32, 1)], [(2, 32, 2), (2, 32, 1)], [(4, 32, 2), (4, 32, 1)],
          [(8, 32, 2), (8, 32, 1)], [(8, 64, 1)], [None], [None], [None]],
 }
-PICKS = (0, 1, 2, 3)
+PICKS = (0, 1, 2, 3, 4)
 
 
 @pytest.mark.parametrize("bits,c", [(8, 3), (8, 4), (16, 3), (16, 4)])

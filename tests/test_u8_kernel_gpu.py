@@ -213,7 +213,7 @@ def test_median_windows(ctx, c):
     if c == 3:
         sets = [[cands[min(pick, len(cands) - 1)] for cands in CANDIDATES[8]] for pick in PICKS]
     else:
-        sets = U8_CHOICES + [SORT_CHOICE]
+        sets = U8_CHOICES + [SORT_CHOICE, [None, None, (1, 64, 1, 1)]]
     for shapes in sets:
         try:
             cpa.tile_shapes_set(ctx, 1, c, shapes)
