@@ -251,15 +251,15 @@ __device__ __forceinline__ void byte_select_median(const uint32_t (&y)[C][ND], u
     }
 }
 
-// The median and both quartiles of every channel's packed bytes: k = {median rank, first
-// quartile rank <= it, third quartile rank >= it}; out and cnt as byte_select_multi's.  The
-// median comes from byte_select_median (real as there).  The first quartile then lies in the
-// 16 values [lo, lo + 15] with lo = max(0, median - 15) when #{y < lo} <= its rank (the
-// window's top, median + 1 or above, already counts more than the median's rank), and the
-// third quartile in [lo, lo + 15] with lo = min(median, 240) when #{y < lo + 16} > its rank
-// (#{y < lo} is at most the median's rank): each takes 4 steps instead of 8.  When the two far-end counts
-// show a quartile outside its window for any lane of the wave, the wave runs the full search
-// for both quartiles instead; either way the values are the exact ranks.
+// The median and both quartiles of every channel's packed bytes: k = {median rank, first quartile
+// rank <= it, third quartile rank >= it}; out and cnt as byte_select_multi's.  The median comes
+// from byte_select_median (real as there).  The first quartile then lies in the 16 values
+// [lo, lo + 15] with lo = max(0, median - 15) when #{y < lo} <= its rank (the window's top,
+// median + 1 or above, already counts more than the median's rank), and the third quartile in
+// [lo, lo + 15] with lo = min(median, 240) when #{y < lo + 16} > its rank (#{y < lo} is at most
+// the median's rank): each takes 4 steps instead of 8.  When the two far-end counts show a
+// quartile outside its window for any lane of the wave, the wave runs the full search for both
+// quartiles instead; either way the values are the exact ranks.
 template <int G, int ND, int C>
 __device__ __forceinline__ void byte_select_quartiles(const uint32_t (&y)[C][ND], const uint32_t (&k)[3], int32_t ntot,
                                                       uint32_t real, uint32_t (&out)[C][3], uint32_t (&cnt)[C][3]) {

@@ -229,3 +229,22 @@ def test_median_windows(ctx, c):
                     assert_same(gpu, ref, f"c={c} n={n} shapes={shapes} abs={absolute}")
         finally:
             cpa.tile_shapes_set(ctx, 1, c, [])
+
+
+@pytest.mark.parametrize("n", [40, 600, 1201, 1279])
+def test_bound_check_partial_lanes(ctx, n):
+    """Lanes whose frame block ends in pads: the bound check that skips outlier-free lanes
+    takes its maxima over the real frames only.  Quiet pixels with a single outlier in the
+    last real frame (often the only real frame of its lane), the first frame or a middle one,
+    under absolute thresholds (every selection mode) on the tall and the short tiles."""
+    rng = np.random.default_rng(4500 + n)
+    h, w, c = 2, 40, 3
+    x = 100 + rng.integers(-1, 2, (n, h, w, c))
+    x[n - 1, :, 0:8] = 250
+    x[0, :, 8:16] = 5
+    x[n // 2, :, 16:24] = 240
+    x[n - 1, 1, 24:32, 1] = 0                            # one channel only
+    x = x.astype(np.uint8)
+    for bg, sel in ((0, 0), (0, 1), (0, 2), (1, 4), (2, 3), (3, 5)):
+        gpu, ref = run_both(ctx, x, True, 0.05, 0.2, bg, sel)
+        assert_same(gpu, ref, f"n={n} bg={bg} sel={sel}")
