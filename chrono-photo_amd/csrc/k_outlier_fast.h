@@ -16,12 +16,13 @@
 //     sum_y |y - v| - sum_y |y - (v-1)| = 2 #{y < v} - n
 // so one search step costs two v_sad_u8 per four byte-samples plus a G-lane DPP reduction.
 // All channels (and, for relative thresholds, all ranks) advance through the same step, so
-// the independent chains interleave.  8-bit stacks need one 8-step level; 16-bit stacks run
-// one level on the high bytes and one on the low bytes clamped into the winning high-byte
-// bucket (values below map to 0, above to 255, which leaves every count #{y < v}, v in
-// [1,255], exact); their medians first try a single 7-step level inside a 128-wide window
-// around an estimate, then the window next to it, and fall back to the two levels when the
-// result lands on a window's edge.  The neighbour order statistic of an even-sized median
+// the independent chains interleave.  8-bit stacks need one 8-step level, which their medians
+// first try as 4 steps in a 16-value window around an estimate and their quartiles as 4 steps
+// next to the median; 16-bit stacks run one level on the high bytes and one on the low bytes
+// clamped into the winning high-byte bucket (values below map to 0, above to 255, which
+// leaves every count #{y < v}, v in [1,255], exact); their medians first try a single 7-step
+// level inside a 128-wide window around an estimate, then the window next to it, and fall
+// back to the two levels when the result lands on a window's edge.  The neighbour order statistic of an even-sized median
 // (and the quartile interpolation partners) is either the same value (when #{x <= v} covers
 // it) or min{x > v} (packed u16 min).  Frames beyond N are padded with the maximum sample
 // value, which never changes the ranks below N.
