@@ -328,7 +328,9 @@ bool read_jpeg(const std::vector<uint8_t>& f, Image* img, std::string* err) {
                 int32_t coef[64] = {};
                 const int t = br.decode(dc[c.td], &ok);
                 if (!ok || t > 11) return (ok = false);
-                c.pred += extend(br.bits(t), t);
+                // a corrupt stream could walk the predictor out of int32 range: valid data stays
+                // within a few thousand, so the clamp changes nothing for it
+                c.pred = std::max(-(1 << 20), std::min(1 << 20, c.pred + extend(br.bits(t), t)));
                 coef[0] = c.pred;
                 for (int k = 1; k < 64;) {
                     const int rs = br.decode(ac[c.ta], &ok);
@@ -344,7 +346,9 @@ bool read_jpeg(const std::vector<uint8_t>& f, Image* img, std::string* err) {
                     coef[kZigzag.z[k]] = extend(br.bits(sz), sz);
                     ++k;
                 }
-                for (int i = 0; i < 64; ++i) F[i] = (float)(coef[i] * (int32_t)qt[c.tq][i]);
+                // the product of two exact floats rounds once, like the float of the integer product,
+                // without the integer overflow a corrupt coefficient could cause
+                for (int i = 0; i < 64; ++i) F[i] = (float)coef[i] * (float)qt[c.tq][i];
                 idct8x8(F, out);
                 for (int y = 0; y < 8; ++y)
                     for (int x = 0; x < 8; ++x)
