@@ -62,6 +62,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(waves_
 outlier_u8_group_kernel(const BatchJobK* __restrict__ jobs, uint32_t njobs) {
     __shared__ TileShared<C, G, F, X> sh;
     __shared__ uint64_t soff[FS > 0 ? G * (FS + 2) : 1];
+    __shared__ float odist[keeps_odist<G, F, X, FS, SORT>() ? (kOdistSlots + 1) * TileShared<C, G, F, X>::P : 1];
     const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
     const BatchJobK& J = jobs[group_job(jobs, njobs, tile)];
     const OutlierK k = global_args(J.k);
@@ -69,7 +70,7 @@ outlier_u8_group_kernel(const BatchJobK* __restrict__ jobs, uint32_t njobs) {
     if constexpr (FS > 0) load_offsets<1, FS>(soff, G * FS, k.samp_off, k.n_sample);
     __syncthreads();
     const uint32_t t = tile - J.tile0;
-    outlier_u8_tile<C, G, F, X, FS, ABS, SORT>(k, t % J.tiles_x, t / J.tiles_x, sh, soff, threadIdx.x);
+    outlier_u8_tile<C, G, F, X, FS, ABS, SORT>(k, t % J.tiles_x, t / J.tiles_x, sh, soff, threadIdx.x, odist);
 }
 
 // The 16-bit-halves kernel over a group of jobs (k_outlier_fast.h).

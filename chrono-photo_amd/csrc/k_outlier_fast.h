@@ -717,11 +717,18 @@ struct StaticFor {
     }
 };
 
+// Slots per pixel of the outlier distances a one-lane-per-pixel tile keeps for the average
+// mode's finish (odist below).
+constexpr int kOdistSlots = 16;
+
 // Finishes pixel p of a tile from the lanes' summaries in LDS: background and outlier
-// selection / blending (src/chrono.rs:290-493), output and statistics.
+// selection / blending (src/chrono.rs:290-493), output and statistics.  odist (nullable):
+// the classification's squared distances of pixel p's outliers, slot s * P + p holding the
+// s-th outlier counted from the last frame, for up to kOdistSlots outliers; with it the
+// average mode takes the distances from there instead of re-reading the samples.
 template <int B, int C, int G, int F, int X, bool ABS>
 __device__ __forceinline__ void finish_pixel(const OutlierK& k, uint32_t bx, uint32_t y, uint32_t p,
-                                             const TileShared<C, G, F, X>& sh) {
+                                             const TileShared<C, G, F, X>& sh, const float* odist = nullptr) {
     using TS = TileShared<C, G, F, X>;
     constexpr int P = TS::P;
     constexpr int MW = TS::MW;
@@ -897,17 +904,22 @@ __device__ __forceinline__ void finish_pixel(const OutlierK& k, uint32_t bx, uin
             float dist;
             if (k.outlier == 3) {
                 float mean_dist = 0.0f;
+                if (odist != nullptr && num <= (uint32_t)kOdistSlots) {
+                    // frame order: the outlier of the lowest frame sits in the last slot
+                    for (uint32_t i = num; i-- > 0;) mean_dist += sqrtf(odist[i * P + p]);
+                } else {
 #pragma unroll
-                for (int q = 0; q < G; ++q) {
+                    for (int q = 0; q < G; ++q) {
 #pragma unroll
-                    for (int w = 0; w < MW; ++w) {
-                        uint32_t mm = gm[q][w];
-                        while (mm) {
-                            const uint32_t b = __builtin_ctz(mm);
-                            mm &= mm - 1;
-                            const uint32_t t = (uint32_t)q * F + (uint32_t)w * 32 + b;
-                            uint32_t tmp[4];
-                            mean_dist += sqrtf(dist_of(t, tmp));
+                        for (int w = 0; w < MW; ++w) {
+                            uint32_t mm = gm[q][w];
+                            while (mm) {
+                                const uint32_t b = __builtin_ctz(mm);
+                                mm &= mm - 1;
+                                const uint32_t t = (uint32_t)q * F + (uint32_t)w * 32 + b;
+                                uint32_t tmp[4];
+                                mean_dist += sqrtf(dist_of(t, tmp));
+                            }
                         }
                     }
                 }

@@ -319,12 +319,24 @@ __device__ __forceinline__ void ranks_sort(const uint32_t (&y)[C][F / 4], const 
     }
 }
 
+// Whether a tile keeps its outliers' squared distances in LDS for the average mode's finish
+// (finish_pixel's odist): one lane per pixel, so the lane that classifies a pixel is the lane
+// that finishes it, and 64-frame lanes, whose pixels have outliers enough for the re-reads
+// to cost more than the stores (the LDS it takes, 16 KB, would cut the occupancy of the
+// shorter one-lane tiles).
+template <int G, int F, int X, int FS, bool SORT>
+constexpr bool keeps_odist() {
+    return SORT && G == 1 && X == 1 && F == 64 && FS == 0;
+}
+
 // One tile: P = X * 256 / G pixels of row y from pixel bx * P.  sh.off holds the tile's frame
 // offsets (all positions) and soff the sample positions' offsets when FS > 0; every lane of
-// the workgroup calls this (it synchronises once).  lane: threadIdx.x.
+// the workgroup calls this (it synchronises once).  lane: threadIdx.x.  odist: (kOdistSlots + 1)
+// * P floats of LDS when keeps_odist(), else unused.
 template <int C, int G, int F, int X, int FS, bool ABS, bool SORT = false>
 __device__ __forceinline__ void outlier_u8_tile(const OutlierK& k, uint32_t bx, uint32_t y,
-                                                TileShared<C, G, F, X>& sh, const uint64_t* soff, uint32_t lane) {
+                                                TileShared<C, G, F, X>& sh, const uint64_t* soff, uint32_t lane,
+                                                float* odist = nullptr) {
     static_assert(F % 8 == 0 && F <= 96 && G <= 32 && FS % 4 == 0 && (X == 1 || X == 4), "tile shape");
     static_assert(!SORT || (G == 1 && X == 1 && FS == 0), "sorting-network ranks: one lane per pixel");
     using TS = TileShared<C, G, F, X>;
@@ -410,7 +422,11 @@ __device__ __forceinline__ void outlier_u8_tile(const OutlierK& k, uint32_t bx, 
 
         // Frames run from last to first so that the mask shifts in bit j for frame j; the
         // largest distance keeps the lowest frame among ties (>=).  lmax starts at the
-        // threshold, so `d >= lmax` also implies that the frame is an outlier.
+        // threshold, so `d >= lmax` also implies that the frame is an outlier.  With
+        // keeps_odist() the average mode's outlier distances go to LDS in that order.
+        constexpr bool ODIST = keeps_odist<G, F, X, FS, SORT>();
+        const bool store_d = ODIST && k.outlier == 3;
+        uint32_t nst = 0;
         auto classify = [&](auto e_t, auto unit_t, auto pos_t, auto max_t, auto sums_t, auto check_t) {
             constexpr int E = decltype(e_t)::value;
             constexpr bool UNIT = decltype(unit_t)::value, POS = decltype(pos_t)::value;
@@ -442,6 +458,12 @@ __device__ __forceinline__ void outlier_u8_tile(const OutlierK& k, uint32_t bx, 
                 if constexpr (SUMS && CHECK) {
 #pragma unroll
                     for (int c = 0; c < C; ++c) sum_out[E][c] += (o && in) ? xs[c] : 0u;
+                }
+                if constexpr (ODIST && SUMS) {
+                    // no branch: frames that are not stored go to the spare slot kOdistSlots
+                    const bool put = store_d && o && in;
+                    odist[(put ? min(nst, (uint32_t)kOdistSlots) : (uint32_t)kOdistSlots) * (uint32_t)P + lane] = d;
+                    nst += put ? 1u : 0u;
                 }
             }
             if constexpr (SUMS && !CHECK) {
@@ -561,7 +583,8 @@ __device__ __forceinline__ void outlier_u8_tile(const OutlierK& k, uint32_t bx, 
         }
     }
     __syncthreads();
-    for (uint32_t p = lane; p < (uint32_t)P; p += 256) finish_pixel<1, C, G, F, X, ABS>(k, bx, y, p, sh);
+    for (uint32_t p = lane; p < (uint32_t)P; p += 256)
+        finish_pixel<1, C, G, F, X, ABS>(k, bx, y, p, sh, keeps_odist<G, F, X, FS, SORT>() ? odist : nullptr);
 }
 
 // VGPRs the frame blocks take, for the occupancy hint: X * C * (F + FS) / 4, plus F for the
@@ -576,11 +599,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(waves_
 outlier_u8_kernel(OutlierK k, uint32_t tiles_x) {
     __shared__ TileShared<C, G, F, X> sh;
     __shared__ uint64_t soff[FS > 0 ? G * (FS + 2) : 1];
+    __shared__ float odist[keeps_odist<G, F, X, FS, SORT>() ? (kOdistSlots + 1) * TileShared<C, G, F, X>::P : 1];
     const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
     load_offsets<1, F>(sh.off, G * F, k.pos_off, k.n);
     if constexpr (FS > 0) load_offsets<1, FS>(soff, G * FS, k.samp_off, k.n_sample);
     __syncthreads();
-    outlier_u8_tile<C, G, F, X, FS, ABS, SORT>(k, tile % tiles_x, tile / tiles_x, sh, soff, threadIdx.x);
+    outlier_u8_tile<C, G, F, X, FS, ABS, SORT>(k, tile % tiles_x, tile / tiles_x, sh, soff, threadIdx.x, odist);
 }
 
 // Whether the packed-byte tiles may run over this composite: every lane's load of a frame
