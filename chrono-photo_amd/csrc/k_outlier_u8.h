@@ -6,7 +6,7 @@
 // the G adjacent lanes of a pixel group each hold a contiguous block of F frames of the
 // group's X pixels (C channels each) as packed bytes, y[e * C + c][i] byte j = channel c of
 // pixel e of frame g * F + 4 i + j, so a lane's F frames take X * C * F / 4 VGPRs and the
-// rank search (byte_select_multi) reads them as they lie.  Frames past N are padded with 255,
+// rank searches (byte_select_median, byte_select_quartiles) read them as they lie.  Frames past N are padded with 255,
 // which never changes a rank below N.
 //
 // Loads.  The memory pipeline spends about the same time on a wave's load instruction
@@ -23,7 +23,8 @@
 // F-frame block of all positions.
 //
 // After the ranks each lane classifies its frames in the reference's f32 order
-// (src/chrono.rs:257-288), leaves its outlier bits, largest distance and sums in LDS, and the
+// (src/chrono.rs:257-288), leaves its outlier bits, largest distance and sums in LDS (the
+// 64-frame sorting tile also its outliers' distances, for the average mode), and the
 // per-pixel finish (finish_pixel, shared with the 16-bit kernel) selects the background and
 // blends (src/chrono.rs:290-493).
 #pragma once
