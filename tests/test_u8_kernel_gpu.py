@@ -248,3 +248,34 @@ def test_bound_check_partial_lanes(ctx, n):
     for bg, sel in ((0, 0), (0, 1), (0, 2), (1, 4), (2, 3), (3, 5)):
         gpu, ref = run_both(ctx, x, True, 0.05, 0.2, bg, sel)
         assert_same(gpu, ref, f"n={n} bg={bg} sel={sel}")
+
+
+def test_sorting_tile_stored_distances(ctx):
+    """The 64-frame sorting tile keeps up to 16 outlier distances per pixel for the average
+    mode's finish and re-reads the samples beyond that: columns with exactly k outliers, k
+    from 0 to 30 (half of them far below, half far above a tight majority, so the quartiles
+    stay with the majority and every far frame is an outlier), at frames spread over the
+    stack, for full and partial (48-frame) stacks.  The majority varies in channel 0 only
+    (ten levels, so its quartiles lie 5 apart and no majority frame reaches 2 IQR)."""
+    rng = np.random.default_rng(4600)
+    ks = [0, 1, 2, 15, 16, 17, 18, 30]
+    for n in (64, 48):
+        h, w, c = 2, 8 * len(ks), 3
+        x = np.full((n, h, w, c), 100)
+        x[..., 0] = 96 + rng.integers(0, 10, (n, h, w))
+        for i, kk in enumerate(ks):
+            kk = min(kk, n // 2 - 2)
+            for col in range(8 * i, 8 * i + 8):
+                frames = rng.choice(n, kk, replace=False)
+                x[frames[: kk // 2], :, col] = 5 + rng.integers(0, 3, (kk // 2, h, c))
+                x[frames[kk // 2:], :, col] = 245 + rng.integers(0, 3, (kk - kk // 2, h, c))
+        x = x.astype(np.uint8)
+        try:
+            cpa.tile_shapes_set(ctx, 1, c, [], [None, None, (1, 64, 1, 2)])
+            for bg in (0, 2, 3):
+                gpu, ref = run_both(ctx, x, False, 2.0, 4.0, bg, 3)
+                assert_same(gpu, ref, f"n={n} bg={bg}")
+                counts = gpu[2]["num_outliers"].reshape(h, w)
+                assert {int(v) for v in counts[0, ::8]} >= {0, 1, 15, 16, 17} if n == 64 else True
+        finally:
+            cpa.tile_shapes_set(ctx, 1, c, [])
