@@ -495,8 +495,15 @@ extern "C" CP_API int cp_autotune(cp_ctx* ctx, uint32_t bps, uint32_t channels, 
             const int* cand = kBucketCand[bps - 1][channels >= 3][b];
             int best = cand[0];
             if (cand[1] >= 0) {
-                float best_ms = 1e30f;
-                for (int c = 0; ok && c < kMaxCand && cand[c] >= 0; ++c) {
+                // two passes over the candidates, the second in reverse order, keeping each
+                // candidate's faster time: a clock still ramping up in the first pass would
+                // otherwise count against the candidates measured first
+                int nc = 0;
+                while (nc < kMaxCand && cand[nc] >= 0) ++nc;
+                float cand_ms[kMaxCand];
+                for (int c = 0; c < nc; ++c) cand_ms[c] = 1e30f;
+                for (int step = 0; ok && step < 2 * nc; ++step) {
+                    const int c = step < nc ? step : 2 * nc - 1 - step;
                     OutlierK k{};
                     k.stack = st->dev;
                     k.pos_off = d_pos;
@@ -520,11 +527,14 @@ extern "C" CP_API int cp_autotune(cp_ctx* ctx, uint32_t bps, uint32_t channels, 
                     ok = ok && hipEventRecord(e1, s) == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
                     float ms = 0.0f;
                     ok = ok && hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
-                    if (ok && ms < best_ms) {
-                        best_ms = ms;
+                    if (ok) cand_ms[c] = std::min(cand_ms[c], ms);
+                }
+                float best_ms = 1e30f;
+                for (int c = 0; c < nc; ++c)
+                    if (cand_ms[c] < best_ms) {
+                        best_ms = cand_ms[c];
                         best = cand[c];
                     }
-                }
             }
             if (!ok) break;
             ctx->tuned[bps - 1][channels - 1][kind][b] = (int8_t)best;
