@@ -882,14 +882,32 @@ __device__ __forceinline__ void finish_pixel(const OutlierK& k, uint32_t bx, uin
                         const int w = FWD ? ww : MW - 1 - ww;
                         uint32_t mm = gm[q][w];
                         while (mm) {
-                            const uint32_t b = FWD ? (uint32_t)__builtin_ctz(mm) : 31 - __builtin_clz(mm);
-                            mm &= ~(1u << b);
-                            const uint32_t t = (uint32_t)q * F + (uint32_t)w * 32 + b;
-                            const float d = dist_of(t, s);
-                            const float fade = fade_pick(k.fade, (int32_t)t, (int32_t)N, k.frame_offset);
-                            const float blend = fade * blend_value(k.th, sqrtf(d));
-                            blend_into_f32<C>(pix_new, s, blend);
-                            blend_inv *= 1.0f - blend;
+                            // up to four outliers at a time: all their samples are requested
+                            // before the first is blended, so the re-reads' latencies overlap;
+                            // the blends still run one after another in frame order
+                            constexpr int KB = 4;
+                            uint32_t tb[KB], sb[KB][4];
+                            bool vb[KB];
+#pragma unroll
+                            for (int u = 0; u < KB; ++u) {
+                                vb[u] = mm != 0u;
+                                const uint32_t b =
+                                    vb[u] ? (FWD ? (uint32_t)__builtin_ctz(mm) : 31 - __builtin_clz(mm)) : 0u;
+                                if (vb[u]) mm &= ~(1u << b);
+                                tb[u] = vb[u] ? (uint32_t)q * F + (uint32_t)w * 32 + b : tb[0];
+                            }
+#pragma unroll
+                            for (int u = 0; u < KB; ++u) sample(tb[u], sb[u]);
+#pragma unroll
+                            for (int u = 0; u < KB; ++u) {
+                                if (!vb[u]) break;
+                                const uint32_t t = tb[u];
+                                const float d = frame_dist<C>(sb[u], med, wi, sg);
+                                const float fade = fade_pick(k.fade, (int32_t)t, (int32_t)N, k.frame_offset);
+                                const float blend = fade * blend_value(k.th, sqrtf(d));
+                                blend_into_f32<C>(pix_new, sb[u], blend);
+                                blend_inv *= 1.0f - blend;
+                            }
                         }
                     }
                 }
@@ -913,12 +931,24 @@ __device__ __forceinline__ void finish_pixel(const OutlierK& k, uint32_t bx, uin
 #pragma unroll
                         for (int w = 0; w < MW; ++w) {
                             uint32_t mm = gm[q][w];
-                            while (mm) {
-                                const uint32_t b = __builtin_ctz(mm);
-                                mm &= mm - 1;
-                                const uint32_t t = (uint32_t)q * F + (uint32_t)w * 32 + b;
-                                uint32_t tmp[4];
-                                mean_dist += sqrtf(dist_of(t, tmp));
+                            while (mm) {  // four re-reads in flight at a time, summed in frame order
+                                constexpr int KB = 4;
+                                uint32_t tb[KB], sb[KB][4];
+                                bool vb[KB];
+#pragma unroll
+                                for (int u = 0; u < KB; ++u) {
+                                    vb[u] = mm != 0u;
+                                    const uint32_t b = vb[u] ? (uint32_t)__builtin_ctz(mm) : 0u;
+                                    mm &= vb[u] ? mm - 1 : mm;
+                                    tb[u] = vb[u] ? (uint32_t)q * F + (uint32_t)w * 32 + b : tb[0];
+                                }
+#pragma unroll
+                                for (int u = 0; u < KB; ++u) sample(tb[u], sb[u]);
+#pragma unroll
+                                for (int u = 0; u < KB; ++u) {
+                                    if (!vb[u]) break;
+                                    mean_dist += sqrtf(frame_dist<C>(sb[u], med, wi, sg));
+                                }
                             }
                         }
                     }
