@@ -734,6 +734,9 @@ __device__ __forceinline__ void finish_pixel(const OutlierK& k, uint32_t bx, uin
     constexpr int MW = TS::MW;
     constexpr uint32_t MAXV = B == 1 ? 255u : 65535u;
     constexpr uint32_t NONE = 0xFFFFFFFFu;
+    // outliers whose samples the finish re-reads together; the four-pixel-per-lane tiles
+    // re-read one at a time (a batch pushes their kernels into private memory)
+    constexpr int KB = X >= 4 ? 1 : 4;
     const uint32_t N = k.n;
     const uint32_t x = bx * P + p;
     if (x >= k.out_w) return;
@@ -882,10 +885,9 @@ __device__ __forceinline__ void finish_pixel(const OutlierK& k, uint32_t bx, uin
                         const int w = FWD ? ww : MW - 1 - ww;
                         uint32_t mm = gm[q][w];
                         while (mm) {
-                            // up to four outliers at a time: all their samples are requested
+                            // up to KB outliers at a time: all their samples are requested
                             // before the first is blended, so the re-reads' latencies overlap;
                             // the blends still run one after another in frame order
-                            constexpr int KB = 4;
                             uint32_t tb[KB], sb[KB][4];
                             bool vb[KB];
 #pragma unroll
@@ -900,13 +902,14 @@ __device__ __forceinline__ void finish_pixel(const OutlierK& k, uint32_t bx, uin
                             for (int u = 0; u < KB; ++u) sample(tb[u], sb[u]);
 #pragma unroll
                             for (int u = 0; u < KB; ++u) {
-                                if (!vb[u]) break;
-                                const uint32_t t = tb[u];
-                                const float d = frame_dist<C>(sb[u], med, wi, sg);
-                                const float fade = fade_pick(k.fade, (int32_t)t, (int32_t)N, k.frame_offset);
-                                const float blend = fade * blend_value(k.th, sqrtf(d));
-                                blend_into_f32<C>(pix_new, sb[u], blend);
-                                blend_inv *= 1.0f - blend;
+                                if (vb[u]) {  // a guard, not a break: the loop stays unrolled
+                                    const uint32_t t = tb[u];
+                                    const float d = frame_dist<C>(sb[u], med, wi, sg);
+                                    const float fade = fade_pick(k.fade, (int32_t)t, (int32_t)N, k.frame_offset);
+                                    const float blend = fade * blend_value(k.th, sqrtf(d));
+                                    blend_into_f32<C>(pix_new, sb[u], blend);
+                                    blend_inv *= 1.0f - blend;
+                                }
                             }
                         }
                     }
@@ -931,8 +934,7 @@ __device__ __forceinline__ void finish_pixel(const OutlierK& k, uint32_t bx, uin
 #pragma unroll
                         for (int w = 0; w < MW; ++w) {
                             uint32_t mm = gm[q][w];
-                            while (mm) {  // four re-reads in flight at a time, summed in frame order
-                                constexpr int KB = 4;
+                            while (mm) {  // KB re-reads in flight at a time, summed in frame order
                                 uint32_t tb[KB], sb[KB][4];
                                 bool vb[KB];
 #pragma unroll
@@ -945,10 +947,8 @@ __device__ __forceinline__ void finish_pixel(const OutlierK& k, uint32_t bx, uin
 #pragma unroll
                                 for (int u = 0; u < KB; ++u) sample(tb[u], sb[u]);
 #pragma unroll
-                                for (int u = 0; u < KB; ++u) {
-                                    if (!vb[u]) break;
-                                    mean_dist += sqrtf(frame_dist<C>(sb[u], med, wi, sg));
-                                }
+                                for (int u = 0; u < KB; ++u)
+                                    if (vb[u]) mean_dist += sqrtf(frame_dist<C>(sb[u], med, wi, sg));
                             }
                         }
                     }
