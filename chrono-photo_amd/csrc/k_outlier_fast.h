@@ -902,14 +902,13 @@ __device__ __forceinline__ void finish_pixel(const OutlierK& k, uint32_t bx, uin
                             for (int u = 0; u < KB; ++u) sample(tb[u], sb[u]);
 #pragma unroll
                             for (int u = 0; u < KB; ++u) {
-                                if (vb[u]) {  // a guard, not a break: the loop stays unrolled
-                                    const uint32_t t = tb[u];
-                                    const float d = frame_dist<C>(sb[u], med, wi, sg);
-                                    const float fade = fade_pick(k.fade, (int32_t)t, (int32_t)N, k.frame_offset);
-                                    const float blend = fade * blend_value(k.th, sqrtf(d));
-                                    blend_into_f32<C>(pix_new, sb[u], blend);
-                                    blend_inv *= 1.0f - blend;
-                                }
+                                if (!vb[u]) break;
+                                const uint32_t t = tb[u];
+                                const float d = frame_dist<C>(sb[u], med, wi, sg);
+                                const float fade = fade_pick(k.fade, (int32_t)t, (int32_t)N, k.frame_offset);
+                                const float blend = fade * blend_value(k.th, sqrtf(d));
+                                blend_into_f32<C>(pix_new, sb[u], blend);
+                                blend_inv *= 1.0f - blend;
                             }
                         }
                     }
@@ -947,8 +946,10 @@ __device__ __forceinline__ void finish_pixel(const OutlierK& k, uint32_t bx, uin
 #pragma unroll
                                 for (int u = 0; u < KB; ++u) sample(tb[u], sb[u]);
 #pragma unroll
-                                for (int u = 0; u < KB; ++u)
-                                    if (vb[u]) mean_dist += sqrtf(frame_dist<C>(sb[u], med, wi, sg));
+                                for (int u = 0; u < KB; ++u) {
+                                    if (!vb[u]) break;
+                                    mean_dist += sqrtf(frame_dist<C>(sb[u], med, wi, sg));
+                                }
                             }
                         }
                     }
