@@ -424,6 +424,45 @@ def test_every_candidate_tile_shape_all_modes(ctx, bits):
         cpa.tile_shapes_set(ctx, bits // 8, 3, [])
 
 
+@pytest.mark.parametrize("bits", [8, 16])
+def test_finish_outlier_counts_every_shape(ctx, bits):
+    """The finish re-reads the outliers' samples up to four at a time before it blends them
+    in frame order (forward, backward) or sums their distances (average): columns with
+    exactly k outliers (k = 0-5, 7-9, 16, 17, 23, at most a third of the stack), whole
+    batches and partial last batches, at random frames and at distances between the two
+    limits, so every blend is partial and the frame order shows in the image, under every
+    candidate tile shape (the 600-frame 8-bit stack runs the four-pixel-per-lane tiles)."""
+    hi = (1 << bits) - 1
+    ks = [0, 1, 2, 3, 4, 5, 7, 8, 9, 16, 17, 23]
+    rng = np.random.default_rng(5100 + bits)
+    stacks = []
+    for n in (32, 64, 128) + ((600,) if bits == 8 else (256,)):
+        h, w = 2, 4 * len(ks)
+        x = np.full((n, h, w, 3), hi // 2, dtype=np.int64)
+        x += rng.integers(-(hi // 100) - 1, hi // 100 + 2, x.shape)
+        want = np.zeros((h, w), dtype=np.int64)
+        for i, kk in enumerate(ks):
+            kk = min(kk, n // 3)
+            for col in range(4 * i, 4 * i + 4):
+                frames = rng.choice(n, kk, replace=False)
+                off = rng.uniform(0.06, 0.16, (kk, h, 1)) * hi * rng.choice([-1, 1], (kk, h, 1))
+                x[frames, :, col] = np.clip(hi // 2 + off, 0, hi).astype(np.int64)
+                want[:, col] = kk
+        stacks.append((x.astype(np.uint8 if bits == 8 else np.uint16), want))
+    try:
+        for pick in PICKS:
+            shapes = [cands[min(pick, len(cands) - 1)] for cands in CANDIDATES[bits]]
+            cpa.tile_shapes_set(ctx, bits // 8, 3, shapes)
+            for x, want in stacks:
+                for bg, sel in ((0, 4), (0, 5), (0, 3), (3, 4), (2, 3)):
+                    gpu, ref = run_both(ctx, x, True, 0.05, 0.2, bg, sel)
+                    msg = f"bits={bits} n={x.shape[0]} pick={pick} bg={bg} sel={sel}"
+                    assert_same(gpu, ref, msg)
+                    assert np.array_equal(gpu[2]["num_outliers"].reshape(want.shape), want), msg
+    finally:
+        cpa.tile_shapes_set(ctx, bits // 8, 3, [])
+
+
 def test_random_configurations(ctx):
     """300 configurations drawn from a fixed seed: sample width, 1-4 channels, 1-300 frames,
     image shape, threshold kind and limits, background and selection mode, weights, fade,
