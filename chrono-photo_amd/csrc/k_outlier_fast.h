@@ -734,9 +734,10 @@ __device__ __forceinline__ void finish_pixel(const OutlierK& k, uint32_t bx, uin
     constexpr int MW = TS::MW;
     constexpr uint32_t MAXV = B == 1 ? 255u : 65535u;
     constexpr uint32_t NONE = 0xFFFFFFFFu;
-    // outliers whose samples the finish re-reads together; the four-pixel-per-lane tiles
-    // re-read one at a time (a batch pushes their kernels into private memory)
-    constexpr int KB = X >= 4 ? 1 : 4;
+    // outliers whose samples the finish re-reads together: eight for the two-pixel tiles
+    // (same registers as four, 0.4% faster on the 4K stack), one for the four-pixel-per-lane
+    // tiles (a batch pushes their kernels into private memory)
+    constexpr int KB = X >= 4 ? 1 : (X == 2 ? 8 : 4);
     const uint32_t N = k.n;
     const uint32_t x = bx * P + p;
     if (x >= k.out_w) return;
